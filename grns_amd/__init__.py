@@ -1,0 +1,287 @@
+"""grns_amd -- Python binding of libgrns_hip.so, the MI355X (gfx950) implementation of the GRNS
+data-parallel RNS hot path.
+
+The binding calls the C ABI of include/grns.h through ctypes.  PyTorch is used only to hold
+device memory and streams (torch.cuda on ROCm is HIP).  Residue arrays are int32 tensors of
+shape (n, count) in modulus-major layout unless strides are given.
+
+There is no fallback: if the shared library is missing the import fails, and every operation
+runs the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .moduli import SMALL_SET, largest_primes_below, params_header, product, workload_set  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgrns_hip.so")
+
+GRNS_OK = 0
+STATUS_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_OOM", 4: "ERR_UNSUPPORTED", 5: "ERR_ADMISSION",
+                6: "ERR_BUSY"}
+
+ER_DTYPE = np.dtype([("frac", "<f8"), ("exp", "<i8")])
+INTERVAL_DTYPE = np.dtype([("low", ER_DTYPE), ("upp", ER_DTYPE)])
+
+
+class GrnsError(RuntimeError):
+    def __init__(self, code, where):
+        super().__init__(f"{where}: {STATUS_NAMES.get(code, code)}")
+        self.code = code
+
+
+class Er(C.Structure):
+    _fields_ = [("frac", C.c_double), ("exp", C.c_longlong)]
+
+
+class Interval(C.Structure):
+    _fields_ = [("low", Er), ("upp", Er)]
+
+
+class MpintArray(C.Structure):
+    _fields_ = [("digits", C.c_void_p), ("sign", C.c_void_p), ("eval", C.c_void_p), ("ld", C.c_longlong)]
+
+
+_lib = None
+
+EXPORTS = [
+    "grns_version", "grns_ctx_create", "grns_ctx_destroy", "grns_ctx_n", "grns_ctx_log2m", "grns_ctx_constants",
+    "grns_ctx_tables", "grns_ctx_status", "grns_eval", "grns_cmp", "grns_div", "grns_scale2pow",
+    "grns_max_workspace_size", "grns_max", "grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul",
+]
+DEBUG_EXPORTS = [
+    "grns_debug_eval_hot", "grns_debug_filter", "grns_debug_ceil_log2", "grns_debug_mulmod_recip_d",
+    "grns_debug_mulmod_recip", "grns_debug_addmod_recip", "grns_debug_bn_err", "grns_debug_fast_scalbn",
+    "grns_debug_filter_tau", "grns_debug_filter_err",
+]
+
+
+def lib():
+    """The loaded libgrns_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m grns_amd.build` "
+                          "(or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    P, I, LL, SZ, D = C.c_void_p, C.c_int, C.c_longlong, C.c_size_t, C.c_double
+    L.grns_version.restype = C.c_char_p
+    L.grns_ctx_create.argtypes = [P, I, I, C.POINTER(P)]
+    L.grns_ctx_destroy.argtypes = [P]
+    L.grns_ctx_n.argtypes = [P]
+    L.grns_ctx_log2m.argtypes = [P]
+    L.grns_ctx_constants.argtypes = [P, P, P, P, P, P, P, P]
+    L.grns_ctx_tables.argtypes = [P, P, P]
+    L.grns_ctx_status.argtypes = [P, P]
+    L.grns_eval.argtypes = [P, P, LL, LL, SZ, P, I, P]
+    L.grns_cmp.argtypes = [P, P, LL, LL, P, LL, LL, SZ, P, P]
+    L.grns_div.argtypes = [P, P, P, LL, LL, SZ, P, P, I, P]
+    L.grns_scale2pow.argtypes = [P, P, LL, LL, P, SZ, P, P]
+    L.grns_max_workspace_size.restype = SZ
+    L.grns_max_workspace_size.argtypes = [P, SZ]
+    L.grns_max.argtypes = [P, P, LL, LL, SZ, P, P, P, SZ, P]
+    for nm in ("grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul"):
+        getattr(L, nm).argtypes = [P, C.POINTER(MpintArray), C.POINTER(MpintArray), C.POINTER(MpintArray), SZ, P, P]
+    L.grns_debug_eval_hot.argtypes = [P, P, P, I]
+    L.grns_debug_filter.restype = D
+    L.grns_debug_filter.argtypes = [P, P, P, P]
+    L.grns_debug_ceil_log2.argtypes = [D]
+    L.grns_debug_mulmod_recip_d.restype = D
+    L.grns_debug_mulmod_recip_d.argtypes = [D, D, D, D]
+    L.grns_debug_mulmod_recip.argtypes = [I, I, I, D]
+    L.grns_debug_addmod_recip.argtypes = [I, I, I, D]
+    L.grns_debug_bn_err.restype = D
+    L.grns_debug_bn_err.argtypes = [D]
+    L.grns_debug_fast_scalbn.restype = D
+    L.grns_debug_fast_scalbn.argtypes = [D, I]
+    L.grns_debug_filter_tau.restype = D
+    L.grns_debug_filter_tau.argtypes = [P]
+    L.grns_debug_filter_err.restype = D
+    L.grns_debug_filter_err.argtypes = [P]
+    _lib = L
+    return L
+
+
+def _check(code, where):
+    if code != GRNS_OK:
+        raise GrnsError(code, where)
+
+
+def _np_ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _tptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+def mpint_array(digits, sign, ev):
+    """grns_mpint_array view of device tensors: digits (n, ld) int32, sign (count,) int32,
+    eval (count, 2, 2) float64 storage holding (frac, exp-bits) pairs."""
+    return MpintArray(digits.data_ptr(), sign.data_ptr(), ev.data_ptr(), digits.stride(0))
+
+
+class Context:
+    """One moduli set resident on a device (grns_ctx_create).  device=-1 makes a host-only
+    context (constants and host test hooks only)."""
+
+    def __init__(self, moduli, device: int = 0):
+        self.moduli = [int(v) for v in moduli]
+        self.n = len(self.moduli)
+        self.device = device
+        arr = np.ascontiguousarray(self.moduli, dtype=np.int32)
+        h = C.c_void_p()
+        _check(lib().grns_ctx_create(_np_ptr(arr), self.n, device, C.byref(h)), "grns_ctx_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().grns_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ----------------------------------------------------------------- constants
+    @property
+    def log2M(self) -> int:
+        return lib().grns_ctx_log2m(self._h)
+
+    def constants(self) -> dict:
+        n = self.n
+        pi = np.zeros(n, np.int32)
+        rd = np.zeros(n, np.float64)
+        ru = np.zeros(n, np.float64)
+        acc = C.c_double()
+        rf = C.c_int()
+        unit = Interval()
+        inv = Interval()
+        _check(lib().grns_ctx_constants(self._h, _np_ptr(pi), _np_ptr(rd), _np_ptr(ru), C.byref(acc), C.byref(rf),
+                                        C.byref(unit), C.byref(inv)), "grns_ctx_constants")
+        return {
+            "log2M": self.log2M, "part_inv": pi.tolist(), "recip_rd": rd.tolist(), "recip_ru": ru.tolist(),
+            "accuracy": acc.value, "ref_factor": rf.value,
+            "unit": ((unit.low.frac, unit.low.exp), (unit.upp.frac, unit.upp.exp)),
+            "inv_unit": ((inv.low.frac, inv.low.exp), (inv.upp.frac, inv.upp.exp)),
+        }
+
+    def tables(self):
+        n = self.n
+        pow2 = np.zeros((self.log2M + 1) * n, np.int32)
+        mrc = np.zeros(n * n, np.int32)
+        _check(lib().grns_ctx_tables(self._h, _np_ptr(pow2), _np_ptr(mrc)), "grns_ctx_tables")
+        return pow2.reshape(-1, n), mrc.reshape(n, n)
+
+    def status(self) -> int:
+        b = C.c_int()
+        _check(lib().grns_ctx_status(self._h, C.byref(b)), "grns_ctx_status")
+        return b.value
+
+    # ----------------------------------------------------------------- device operations
+    # Residue tensors: int32, shape (n, count) (modulus-major) or any 2-D view whose strides the
+    # kernels can follow.  Outputs are allocated with torch on the tensor's device.
+    @staticmethod
+    def _strides(x):
+        return x.stride(0), x.stride(1)
+
+    def eval(self, x, fast=False, out=None, stream=None):
+        import torch
+        cnt = x.shape[1]
+        if out is None:
+            out = torch.empty((cnt, 4), dtype=torch.float64, device=x.device)
+        sm, sn = self._strides(x)
+        _check(lib().grns_eval(self._h, _tptr(x), sm, sn, cnt, _tptr(out), int(fast), _stream_ptr(stream)),
+               "grns_eval")
+        return out
+
+    def cmp(self, x, y, out=None, stream=None):
+        import torch
+        cnt = x.shape[1]
+        if out is None:
+            out = torch.empty(cnt, dtype=torch.int32, device=x.device)
+        xs, ys = self._strides(x), self._strides(y)
+        _check(lib().grns_cmp(self._h, _tptr(x), xs[0], xs[1], _tptr(y), ys[0], ys[1], cnt, _tptr(out),
+                              _stream_ptr(stream)), "grns_cmp")
+        return out
+
+    def div(self, x, d, fast=True, q=None, r=None, stream=None):
+        import torch
+        if q is None:
+            q = torch.empty_like(x)
+        if r is None:
+            r = torch.empty_like(x)
+        sm, sn = self._strides(x)
+        assert self._strides(d) == (sm, sn) and self._strides(q) == (sm, sn) and self._strides(r) == (sm, sn)
+        _check(lib().grns_div(self._h, _tptr(x), _tptr(d), sm, sn, x.shape[1], _tptr(q), _tptr(r), int(fast),
+                              _stream_ptr(stream)), "grns_div")
+        return q, r
+
+    def scale2pow(self, x, D, y=None, stream=None):
+        import torch
+        if y is None:
+            y = torch.empty_like(x)
+        sm, sn = self._strides(x)
+        assert self._strides(y) == (sm, sn)
+        _check(lib().grns_scale2pow(self._h, _tptr(x), sm, sn, _tptr(D), x.shape[1], _tptr(y),
+                                    _stream_ptr(stream)), "grns_scale2pow")
+        return y
+
+    def max_workspace_size(self, count) -> int:
+        return lib().grns_max_workspace_size(self._h, count)
+
+    def max(self, x, workspace=None, out=None, out_idx=None, stream=None):
+        import torch
+        cnt = x.shape[1]
+        if workspace is None:
+            workspace = torch.empty(self.max_workspace_size(cnt), dtype=torch.uint8, device=x.device)
+        if out is None:
+            out = torch.empty(self.n, dtype=torch.int32, device=x.device)
+        if out_idx is None:
+            out_idx = torch.empty(1, dtype=torch.int64, device=x.device)
+        sm, sn = self._strides(x)
+        _check(lib().grns_max(self._h, _tptr(x), sm, sn, cnt, _tptr(out), _tptr(out_idx), _tptr(workspace),
+                              workspace.numel(), _stream_ptr(stream)), "grns_max")
+        return out, out_idx
+
+    def mpint(self, op, x, y, z, count, overflow=None, stream=None):
+        """op in {'add','sub','mul'}; x, y, z: (digits, sign, eval) device tensor triples."""
+        fn = {"add": lib().grns_mpint_add, "sub": lib().grns_mpint_sub, "mul": lib().grns_mpint_mul}[op]
+        xa, ya, za = mpint_array(*x), mpint_array(*y), mpint_array(*z)
+        _check(fn(self._h, C.byref(xa), C.byref(ya), C.byref(za), count,
+                  _tptr(overflow) if overflow is not None else None, _stream_ptr(stream)), f"grns_mpint_{op}")
+
+    # ----------------------------------------------------------------- host test hooks
+    def debug_eval_hot(self, residues, fast=False):
+        a = np.ascontiguousarray(residues, dtype=np.int32)
+        out = np.zeros(1, INTERVAL_DTYPE)
+        ok = lib().grns_debug_eval_hot(self._h, _np_ptr(a), _np_ptr(out), int(fast))
+        return ok, out[0]
+
+    def debug_filter(self, residues):
+        a = np.ascontiguousarray(residues, dtype=np.int32)
+        rg, cn = C.c_int(), C.c_int()
+        F = lib().grns_debug_filter(self._h, _np_ptr(a), C.byref(rg), C.byref(cn))
+        return F, bool(rg.value), bool(cn.value)
+
+
+def version() -> str:
+    return lib().grns_version().decode()

@@ -1,0 +1,501 @@
+// grns_device.hpp -- device (and host-testable) primitives of the GRNS hot path for gfx950.
+//
+// Every floating-point routine here reproduces the operation sequence of the GRNS 1.1.4 CPU
+// path bit for bit: round-to-nearest FP64 with the BoundNear1 emulation of directed rounding
+// (src/dinterval.cuh:37-147), the reciprocal modular reduction of src/modular.cuh:29-79, the
+// zero-padded pairwise trees of src/pairwise.cuh:121-458 and x86 conversion semantics.  This
+// translation unit is compiled with -ffp-contract=off; the only fused multiply-adds are the
+// explicit fma() calls below, each of which is exact or provably returns the same double as the
+// unfused reference expression (comments at each site).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GRNS_HD __host__ __device__ __forceinline__
+#define GRNS_D __device__ __forceinline__
+
+namespace grns {
+
+// er_float_t / interval_t of src/types.cuh:13-24 (long is 64-bit on the LP64 targets).
+struct er_t {
+    double frac;
+    long long exp;
+};
+struct interval_t {
+    er_t low;
+    er_t upp;
+};
+
+constexpr int MAXN = 256;      // largest moduli set a context accepts
+constexpr int THRESH = 30;     // RNS_P2_SCALING_THRESHOLD, src/params.h:22
+
+// Per-moduli-set constants resident in HBM (rns_const_init, src/rnsbase.cuh:324-451, plus the
+// tables this build's filtered paths need).
+struct DevCtx {
+    int n;
+    int log2M;
+    int ref_factor;
+    int pad0;
+    double accuracy;
+    double tau;        // fraction window outside of which no evaluation needs MRC or refinement
+    double err;        // bound on |approx fraction - x/M mod 1| of the filter sums
+    er_t unit_low, unit_upp, inv_low, inv_upp;
+    int m[MAXN];
+    int w[MAXN];                 // RNS_PART_MODULI_PRODUCT_INVERSE
+    double dm[MAXN];             // (double) m_i
+    double dw[MAXN];             // (double) w_i
+    double dinv[MAXN];           // 1.0 / m_i (round to nearest), the reciprocal of mod_add/mod_mul
+    double rd[MAXN];             // RNS_MODULI_RECIP_RD
+    double ru[MAXN];             // RNS_MODULI_RECIP_RU
+    double chi[MAXN];            // w_i / m_i as a double-double (filter sums)
+    double clo[MAXN];
+    int c32[MAXN];               // 2^32 mod m_i
+    unsigned long long mi64[MAXN];  // (M / m_i) mod 2^64
+    unsigned long long M64;         // M mod 2^64
+    int prod_pow2_res[THRESH];               // RNS_MODULI_PRODUCT_POW2_RESIDUES
+    int part_pow2_res[THRESH][MAXN];         // RNS_PART_MODULI_PRODUCT_POW2_RESIDUES
+    int pow2_inv[THRESH][MAXN];              // RNS_POW2_INVERSE
+    const int *pow2;                         // RNS_POW2, (log2M+1) x n, device memory
+    const int *mrc_inv;                      // MRC_MULT_INV, n x n, device memory
+};
+
+// ---------------------------------------------------------------- BoundNear1 (src/dinterval.cuh)
+constexpr double EPS = 0x1p-53;
+constexpr double ETA = 0x1p-1074;
+constexpr double PHI1 = EPS * (1.0 + 2.0 * EPS);   // 2^-53 (1 + 2^-52), exact
+
+// The reference's fmac(PHI1, |c|, ETA) = RN(RN(PHI1*|c|) + ETA).  One fma returns the same
+// double whenever c == 0 or |c| >= 2^-917: then ETA is below a quarter of the spacing of the
+// grid around PHI1*|c| and PHI1*|c| is never within ETA of a rounding midpoint except the exact
+// tie at c = 1.5 * 2^k, which both forms round upward.  For c a small multiple of ETA both give
+// ETA.  Every c reaching this function on the hot paths is 0, a multiple of ETA below 2^-1000,
+// or has |c| >= 2^-83 (products and sums of residue fractions, extended-range significands).
+GRNS_HD double bn_err(double c) { return fma(PHI1, fabs(c), ETA); }
+
+GRNS_HD double dadd_rd(double a, double b) { double c = a + b; return c - bn_err(c); }
+GRNS_HD double dadd_ru(double a, double b) { double c = a + b; return c + bn_err(c); }
+GRNS_HD double dsub_rd(double a, double b) { double c = a - b; return c - bn_err(c); }
+GRNS_HD double dsub_ru(double a, double b) { double c = a - b; return c + bn_err(c); }
+GRNS_HD double dmul_rd(double a, double b) { double c = a * b; return c - bn_err(c); }
+GRNS_HD double dmul_ru(double a, double b) { double c = a * b; return c + bn_err(c); }
+GRNS_HD double ddiv_rd(double a, double b) { double c = a / b; return c - bn_err(c); }
+GRNS_HD double ddiv_ru(double a, double b) { double c = a / b; return c + bn_err(c); }
+
+template <bool UP> GRNS_HD double dadd_dir(double a, double b) { return UP ? dadd_ru(a, b) : dadd_rd(a, b); }
+
+GRNS_HD uint64_t dbits(double x) {
+    union { double d; uint64_t u; } v; v.d = x; return v.u;
+}
+GRNS_HD double bitsd(uint64_t u) {
+    union { double d; uint64_t u; } v; v.u = u; return v.d;
+}
+
+// fast_scalbn, src/bitwise.cuh:75-80: add n to the exponent field, then multiply by 0 or 1.
+// The multiply by zero of an infinity or NaN produced by the exponent wrap is spelled out with
+// the x86 SSE results (default NaN 0xFFF8..., input NaN quieted), so that device and CPU agree.
+GRNS_HD double fast_scalbn(double x, int n) {
+    uint64_t u = dbits(x) + ((uint64_t)(int64_t)n << 52);
+    double v = bitsd(u);
+    if (x != 0 && n >= -1023) return v;
+    uint64_t e = (u >> 52) & 0x7FF;
+    if (e == 0x7FF) {
+        if ((u & ((1ull << 52) - 1)) == 0) return bitsd(0xFFF8000000000000ull);  // inf * 0
+        return bitsd(u | (1ull << 51));                                            // NaN * 0
+    }
+    return bitsd(u & (1ull << 63));  // signed zero
+}
+
+// x86-64 (unsigned int)double (cvttsd2si to 64 bits, low 32 kept), for |x| < 2^31.
+GRNS_HD unsigned to_uint(double x) { return (unsigned)(int)x; }
+
+// ---------------------------------------------------------------- extended range (src/extrange.cuh)
+GRNS_HD er_t er_set_d(double x) {
+    er_t r;
+    uint64_t u = dbits(x);
+    if (x != 0) {
+        r.exp = (long long)(((u & ~(1ull << 63)) >> 52) - 1023ull);
+        r.frac = bitsd((u & (((1ull << 52) - 1) | (1ull << 63))) | (1023ull << 52));
+    } else {
+        r.exp = 0;
+        r.frac = 0;
+    }
+    return r;
+}
+// src/extrange.cuh:116-125
+GRNS_HD er_t er_adjust(er_t x) {
+    uint64_t u = dbits(x.frac);
+    bool nz = x.frac != 0;
+    er_t r;
+    r.exp = (long long)(((uint64_t)x.exp + ((u & ~(1ull << 63)) >> 52) - 1023ull) * (uint64_t)nz);
+    double f = bitsd((u & (((1ull << 52) - 1) | (1ull << 63))) | (1023ull << 52));
+    r.frac = f * (double)(nz ? 1 : 0);
+    return r;
+}
+// src/extrange.cuh:153-248; the reference resets the inputs' exponents when their fraction is 0.
+template <bool SUB, bool UP>
+GRNS_HD er_t er_addsub(er_t &x, er_t &y) {
+    x.exp *= (x.frac != 0);
+    y.exp *= (y.frac != 0);
+    int dexp = (int)((x.exp - y.exp) * (long long)(x.frac != 0) * (long long)(y.frac != 0));
+    er_t r;
+    if (dexp > 0) {
+        r.exp = x.exp;
+        double ys = fast_scalbn(y.frac, -dexp);
+        r.frac = SUB ? (UP ? dsub_ru(x.frac, ys) : dsub_rd(x.frac, ys)) : (UP ? dadd_ru(x.frac, ys) : dadd_rd(x.frac, ys));
+    } else if (dexp < 0) {
+        r.exp = y.exp;
+        double xs = fast_scalbn(x.frac, dexp);
+        r.frac = SUB ? (UP ? dsub_ru(xs, y.frac) : dsub_rd(xs, y.frac)) : (UP ? dadd_ru(y.frac, xs) : dadd_rd(y.frac, xs));
+    } else {
+        r.exp = (x.exp == 0) ? y.exp : x.exp;
+        r.frac = SUB ? (UP ? dsub_ru(x.frac, y.frac) : dsub_rd(x.frac, y.frac)) : (UP ? dadd_ru(x.frac, y.frac) : dadd_rd(x.frac, y.frac));
+    }
+    return er_adjust(r);
+}
+// src/extrange.cuh:289-320
+GRNS_HD er_t er_div_rd(const er_t &x, const er_t &y) {
+    er_t r; r.exp = x.exp - y.exp; r.frac = ddiv_rd(x.frac, y.frac); return er_adjust(r);
+}
+GRNS_HD er_t er_md_rd(const er_t &x, const er_t &y, const er_t &z) {
+    er_t r; r.exp = x.exp + y.exp - z.exp; r.frac = ddiv_rd(dmul_rd(x.frac, y.frac), z.frac); return er_adjust(r);
+}
+GRNS_HD er_t er_md_ru(const er_t &x, const er_t &y, const er_t &z) {
+    er_t r; r.exp = x.exp + y.exp - z.exp; r.frac = ddiv_ru(dmul_ru(x.frac, y.frac), z.frac); return er_adjust(r);
+}
+// src/extrange.cuh:347-353
+GRNS_HD int er_ucmp(const er_t &x, const er_t &y) {
+    int diff = (int)(x.exp - y.exp);
+    if (diff == 0) return (x.frac == y.frac ? 0 : x.frac > y.frac ? 1 : -1);
+    return x.frac != 0 && (diff > 0 || (diff < 0 && y.frac == 0)) ? 1 : -1;
+}
+
+// ---------------------------------------------------------------- modular (src/modular.cuh)
+// mod_mul(x, y, m, 1.0/m) of the CPU path: (int)(r - (int)(RN(RN(r) * d)) * m) with r = x*y.
+// Computed in FP64: RN(x*y) is one multiply, the quotient is truncated as a double, and the
+// difference x*y - i*m is formed exactly by two fmas (|p - i*m| < 2^34).  The int32 wrap of
+// the reference's final cast is applied to the exact value.  Requires |quotient| < 2^31, which
+// holds for residues within (-2^31, 2^31) and a canonical second factor.
+GRNS_HD double mulmod_recip_d(double dx, double dy, double dm, double d) {
+    double p = dx * dy;
+    double di = trunc(p * d);
+    double lo = fma(dx, dy, -p);   // x*y - p, exact
+    double t = fma(-di, dm, p);    // p - i*m, exact
+    double s = t + lo;             // x*y - i*m, exact
+    s = (s >= 2147483648.0) ? s - 4294967296.0 : s;
+    s = (s < -2147483648.0) ? s + 4294967296.0 : s;
+    return s;
+}
+GRNS_HD int mulmod_recip(int x, int y, int m, double d) {
+    double q = ((double)x * (double)y) * d;
+    int i = (int)q;
+    return (int)((unsigned)x * (unsigned)y - (unsigned)i * (unsigned)m);
+}
+// mod_add(x, y, m, 1.0/m), src/modular.cuh:29-34 (x + y is exact in FP64)
+GRNS_HD int addmod_recip(int x, int y, int m, double d) {
+    double q = ((double)x + (double)y) * d;
+    int i = (int)q;
+    return (int)((unsigned)x + (unsigned)y - (unsigned)i * (unsigned)m);
+}
+// C's truncating remainder v % m for |v| < 2^62 (dinv = RN(1/m)).
+GRNS_HD long long rem_trunc(long long v, int m, double dinv) {
+    double q = trunc((double)v * dinv);
+    long long r = v - (long long)q * m;
+    if (v >= 0) {
+        if (r < 0) r += m;
+        else if (r >= m) r -= m;
+    } else {
+        if (r > 0) r -= m;
+        else if (r <= -m) r += m;
+    }
+    return r;
+}
+// 3-argument mod_mul, src/modular.cuh:62-66
+GRNS_HD int mulmod3(int x, int y, int m, double dinv) { return (int)rem_trunc((long long)x * (long long)y, m, dinv); }
+// mod_psub, src/modular.cuh:52-56
+GRNS_HD int psubmod(int x, int y, int m, double dinv) {
+    return (int)rem_trunc((long long)x - (long long)y + (long long)m, m, dinv);
+}
+
+// ceil(log2(s)) as glibc computes it for a positive double s (the refinement factor of
+// src/rnseval.cuh:82).  log2 is exact at powers of two; just above 2^e (e < 0) the correctly
+// rounded result of e + log2(f) is e itself while log2(f) stays below half the spacing of the
+// doubles next to |e|, so the ceiling is then e rather than e + 1.
+GRNS_HD int ceil_log2_libm(double s) {
+    uint64_t u = dbits(s);
+    int ef = (int)((u >> 52) & 0x7FF);
+    uint64_t mant = u & ((1ull << 52) - 1);
+    int e;
+    if (ef == 0) {  // subnormal: normalise
+        int lz = 0;
+        while (!(mant & (1ull << 51))) { mant <<= 1; lz++; }
+        mant = (mant << 1) & ((1ull << 52) - 1);
+        e = -1023 - lz;
+    } else {
+        e = ef - 1023;
+    }
+    if (mant == 0) return e;
+    if (e == 0) return 1;
+    if (mant > (1ull << 40)) return e + 1;  // log2(f) > 2^-12.5, far above any half-spacing
+    double t = (double)mant * 0x1p-52;      // f - 1
+    double l2f = t * (1.4426950408889634 - 0.7213475204444817 * t);  // log2(1 + t), t < 2^-12
+    int ae = e < 0 ? -e : e;
+    int p = 31 - __builtin_clz((unsigned)ae);          // 2^p <= |e| < 2^(p+1)
+    double half = 0x1p-53 * (double)(1u << p);         // half-spacing above |e| (binade of |e|)
+    if (e < 0 && ae == (1 << p)) half *= 0.5;          // |e| a power of two: finer grid below it
+    return (l2f < half) ? e : e + 1;
+}
+
+// ---------------------------------------------------------------- thread-per-number trees
+// psumK<L> of src/pairwise.cuh:227-327 unrolled at compile time over leaf functors; the leaves
+// are produced on demand so only O(log N) partial sums are live.
+template <int L>
+struct TreeWidth {
+    static constexpr int calc() {
+        if (L >= 4096) return 8192;
+        int K = 8;
+        while (K <= L) K <<= 1;
+        return K;
+    }
+    static constexpr int v = calc();
+};
+
+template <int K, int L, int OFF>
+struct PTree {
+    template <class Leaf>
+    GRNS_HD static void run(Leaf &leaf, double &lo, double &up) {
+        if constexpr (K == 2) {
+            double l0, u0;
+            leaf.template get<OFF>(l0, u0);
+            if constexpr (L > 1) {
+                double l1, u1;
+                leaf.template get<OFF + 1>(l1, u1);
+                lo = dadd_rd(l0, l1);
+                up = dadd_ru(u0, u1);
+            } else {
+                lo = dadd_rd(l0, 0.0);
+                up = dadd_ru(u0, 0.0);
+            }
+        } else {
+            double l1, u1, l2 = 0.0, u2 = 0.0;
+            PTree<K / 2, L, OFF>::run(leaf, l1, u1);
+            if constexpr (L > K / 2) PTree<K / 2, L - K / 2, OFF + K / 2>::run(leaf, l2, u2);
+            lo = dadd_rd(l1, l2);
+            up = dadd_ru(u1, u2);
+        }
+    }
+};
+
+// psum_rd<N> and psum_ru<N> together (src/pairwise.cuh:384-458 dispatch).
+template <int N, class Leaf>
+GRNS_HD void psum_pair(Leaf &leaf, double &lo, double &up) {
+    if constexpr (N >= 4) {
+        PTree<TreeWidth<N>::v, N, 0>::run(leaf, lo, up);
+    } else if constexpr (N == 3) {
+        double l0, u0, l1, u1, l2, u2;
+        leaf.template get<0>(l0, u0);
+        leaf.template get<1>(l1, u1);
+        leaf.template get<2>(l2, u2);
+        lo = dadd_rd(dadd_rd(l0, l1), l2);
+        up = dadd_ru(dadd_ru(u0, u1), u2);
+    } else if constexpr (N == 2) {
+        double l0, u0, l1, u1;
+        leaf.template get<0>(l0, u0);
+        leaf.template get<1>(l1, u1);
+        lo = dadd_rd(l0, l1);
+        up = dadd_ru(u0, u1);
+    } else {
+        leaf.template get<0>(lo, up);
+    }
+}
+
+// ---------------------------------------------------------------- wave groups (N lanes per number)
+// A group is G = 2^g >= N consecutive lanes of a wave64, aligned on G; lane gl of a group holds
+// modulus gl.  Cross-lane traffic stays inside the group.
+struct Group {
+    int gl;    // lane within group
+    int base;  // first wave lane of the group
+    int G;     // group width
+    int L;     // number of moduli (active leaves)
+};
+
+GRNS_D unsigned long long group_mask_bits(const Group &g) {
+    return (g.G == 64) ? ~0ull : (((1ull << g.G) - 1) << g.base);
+}
+GRNS_D bool group_all(const Group &g, bool p) {
+    unsigned long long b = __ballot(p);
+    unsigned long long gm = group_mask_bits(g);
+    return (b & gm) == gm;
+}
+GRNS_D bool group_any(const Group &g, bool p) {
+    return (__ballot(p) & group_mask_bits(g)) != 0;
+}
+GRNS_D int bcast_i(int v, int lane) { return __shfl(v, lane, 64); }
+GRNS_D double bcast_d(double v, int lane) { return __shfl(v, lane, 64); }
+GRNS_D long long bcast_ll(long long v, int lane) { return __shfl(v, lane, 64); }
+
+// Pairwise tree over the group's leaves in the CPU association (xor butterfly; an absent second
+// half contributes the literal 0 of src/pairwise.cuh, an absent first half makes the node absent).
+template <bool UP>
+GRNS_D double group_psum(const Group &g, double v) {
+    const int L = g.L;
+    if (L == 1) return bcast_d(v, g.base);
+    if (L == 3) {
+        double a = bcast_d(v, g.base), b = bcast_d(v, g.base + 1), c = bcast_d(v, g.base + 2);
+        return dadd_dir<UP>(dadd_dir<UP>(a, b), c);
+    }
+    int K = 2;
+    if (L >= 4) { K = 8; while (K <= L) K <<= 1; }
+    for (int h = 1; h < K; h <<= 1) {
+        double w = (h < g.G) ? __shfl_xor(v, h, 64) : 0.0;
+        int b = g.gl & ~(2 * h - 1);
+        bool first = b < L;
+        bool second = (h < g.G) && (b + h < L);
+        bool upper = (g.gl & h) != 0;
+        double fv = upper ? w : v;
+        double sv = second ? (upper ? v : w) : 0.0;
+        double r = dadd_dir<UP>(fv, sv);
+        v = first ? r : v;
+    }
+    return v;
+}
+
+GRNS_D long long group_sum_ll(const Group &g, long long v) {
+    for (int h = 1; h < g.G; h <<= 1) v += __shfl_xor(v, h, 64);
+    return v;
+}
+
+// Mixed-radix digits (src/mrc.cuh:31-44), pipelined: at step j the final digit j is broadcast
+// and every higher lane applies the same update the serial loop applies to it.
+GRNS_D int group_mrc(const DevCtx *c, const Group &g, int xi) {
+    int mr = xi;
+    const int n = c->n;
+    const bool act = g.gl < g.L;
+    const int mi = act ? c->m[g.gl] : 1;
+    const double dinv = act ? c->dinv[g.gl] : 1.0;
+    for (int j = 0; j < g.L - 1; j++) {
+        int mrj = bcast_i(mr, g.base + j);
+        if (act && g.gl > j) {
+            if (mr < mrj) mr = (int)((long long)mi - (long long)mrj + (long long)mr);
+            else mr = (int)((unsigned)mr - (unsigned)mrj);
+            mr = mulmod3(mr, c->mrc_inv[j * n + g.gl], mi, dinv);
+        }
+    }
+    return mr;
+}
+// mrs_cmp (src/mrc.cuh:13-22) of two digit vectors held by the group.
+GRNS_D int group_mrs_cmp(const Group &g, int a, int b) {
+    unsigned long long diff = __ballot(g.gl < g.L && a != b) & group_mask_bits(g);
+    if (diff == 0) return 0;
+    int k = 63 - __clzll(diff);
+    int ak = bcast_i(a, k), bk = bcast_i(b, k);
+    return ak > bk ? 1 : -1;
+}
+
+GRNS_D er_t bcast_er(er_t v, int lane) {
+    er_t r;
+    r.frac = bcast_d(v.frac, lane);
+    r.exp = bcast_ll(v.exp, lane);
+    return r;
+}
+
+// Status bits a group computation can raise (cases in which the reference itself would read
+// outside its tables or not terminate).
+enum : int { GST_OK = 0, GST_REFINE = 1, GST_DIVLOOP = 2 };
+
+// Refinement loop, src/rnseval.cuh:79-101 (identical tail of rns_eval_compute_fast).
+GRNS_D void group_refine(const DevCtx *c, const Group &g, int &s, double sumu, interval_t &out, int &status) {
+    const bool act = g.gl < g.L;
+    const int mi = act ? c->m[g.gl] : 1;
+    const double dm = act ? c->dm[g.gl] : 1.0;
+    const double dinv = act ? c->dinv[g.gl] : 1.0;
+    long long K = 0;
+    int it = 0;
+    while (sumu < c->accuracy) {
+        // MAX(-(ceil(log2(sumu))+1), REF): log2 of a negative sum is NaN and selects REF; a zero
+        // sum gives +inf, which the reference turns into an index outside RNS_POW2.
+        double kv = (double)c->ref_factor;
+        if (sumu > 0) {
+            double a = (double)(-(ceil_log2_libm(sumu) + 1));
+            kv = a > kv ? a : kv;
+        }
+        if (sumu == 0 || kv > c->log2M || ++it > 4096) {
+            status |= GST_REFINE;
+            out.low.frac = out.upp.frac = __builtin_nan("");
+            out.low.exp = out.upp.exp = 0;
+            return;
+        }
+        int k = (int)kv;
+        int p2 = act ? c->pow2[(size_t)k * c->n + g.gl] : 0;
+        s = mulmod_recip(s, p2, mi, dinv);
+        double fu = ddiv_ru((double)s, dm);
+        sumu = group_psum<true>(g, fu);
+        sumu -= to_uint(sumu);
+        K += k;
+    }
+    double fl = ddiv_rd((double)s, dm);
+    double suml = group_psum<false>(g, fl);
+    suml -= to_uint(suml);
+    out.low = er_set_d(suml);
+    out.upp = er_set_d(sumu);
+    out.low.exp -= K;
+    out.upp.exp -= K;
+}
+
+// rns_eval_compute (FAST=false, src/rnseval.cuh:29-102) and rns_eval_compute_fast (FAST=true,
+// :112-167) for the number whose residue of modulus gl is xi.  The result is the same on all
+// lanes of the group.
+template <bool FAST>
+GRNS_D interval_t group_eval(const DevCtx *c, const Group &g, int xi, int &status) {
+    interval_t out;
+    const bool act = g.gl < g.L;
+    if (group_all(g, !act || xi == 0)) {
+        out.low.frac = 0; out.low.exp = 0; out.upp.frac = 0; out.upp.exp = 0;
+        return out;
+    }
+    int s = act ? mulmod_recip(xi, c->w[g.gl], c->m[g.gl], c->dinv[g.gl]) : 0;
+    double fl = act ? dmul_rd((double)s, c->rd[g.gl]) : 0.0;
+    double fu = act ? dmul_ru((double)s, c->ru[g.gl]) : 0.0;
+    double suml = group_psum<false>(g, fl);
+    double sumu = group_psum<true>(g, fu);
+    unsigned whl = to_uint(suml), whu = to_uint(sumu);
+    if (FAST) {
+        suml -= whl;
+        sumu -= whu;
+        if (sumu >= c->accuracy) {
+            out.low = er_set_d(suml);
+            out.upp = er_set_d(sumu);
+            return out;
+        }
+        group_refine(c, g, s, sumu, out, status);
+        return out;
+    }
+    suml = suml - whl;
+    sumu = sumu - whu;
+    out.low = er_set_d(suml);
+    out.upp = er_set_d(sumu);
+    int mr = -1;
+    if (whl != whu) {
+        int d = group_mrc(c, g, xi);
+        mr = bcast_i(d, g.base + g.L - 1);
+    }
+    if (mr > 0) { out.upp = c->inv_upp; return out; }
+    if (mr == 0) out.low = c->unit_low;
+    if (sumu >= c->accuracy) return out;
+    group_refine(c, g, s, sumu, out, status);
+    return out;
+}
+
+// rns_cmp, src/rnscmp.cuh:20-39
+GRNS_D int group_cmp(const DevCtx *c, const Group &g, int xi, int yi, int &status) {
+    interval_t ex = group_eval<false>(c, g, xi, status);
+    interval_t ey = group_eval<false>(c, g, yi, status);
+    if (er_ucmp(ex.low, ey.upp) > 0) return 1;
+    if (er_ucmp(ey.low, ex.upp) > 0) return -1;
+    if (group_all(g, g.gl >= g.L || xi == yi)) return 0;
+    int mx = group_mrc(c, g, xi);
+    int my = group_mrc(c, g, yi);
+    return group_mrs_cmp(g, mx, my);
+}
+
+}  // namespace grns

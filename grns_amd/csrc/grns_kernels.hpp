@@ -1,0 +1,669 @@
+// grns_kernels.hpp -- HIP kernels of the GRNS hot path for gfx950 (wave64).
+//
+// Two mappings are used:
+//  * thread-per-number for the HBM-bound element-wise work (evaluation, comparison, arg-max,
+//    mpint add/sub/mul): the n residues of one number sit in one lane's registers, the loads of
+//    a modulus-major array are coalesced across the wave, and the pairwise trees are unrolled
+//    at compile time for n in {4, 8, 16, 32, 64};
+//  * group-per-number (G = 2^ceil(log2 n) lanes, modulus i in lane i) for the long, branchy
+//    loops (division, power-of-two scaling) and for the rare cases of the element-wise kernels
+//    (mixed-radix conversion, refinement), which the wave resolves cooperatively after a ballot.
+#pragma once
+
+#include "grns_device.hpp"
+
+namespace grns {
+
+constexpr int BLOCK = 256;
+
+template <int N>
+struct Pow2Ceil {
+    static constexpr int v = N <= 1 ? 1 : (N <= 2 ? 2 : (N <= 4 ? 4 : (N <= 8 ? 8 : (N <= 16 ? 16 : (N <= 32 ? 32 : 64)))));
+};
+
+GRNS_D Group make_group(int G, int L) {
+    int lane = threadIdx.x & 63;
+    Group g;
+    g.G = G;
+    g.gl = lane & (G - 1);
+    g.base = lane & ~(G - 1);
+    g.L = L;
+    return g;
+}
+
+// The k-th set bit of mask (k >= 0), or -1.
+GRNS_D int kth_bit(unsigned long long mask, int k) {
+    for (int i = 0; i < k && mask; i++) mask &= mask - 1;
+    return mask ? (__ffsll((long long)mask) - 1) : -1;
+}
+GRNS_D unsigned long long drop_low_bits(unsigned long long mask, int k) {
+    for (int i = 0; i < k && mask; i++) mask &= mask - 1;
+    return mask;
+}
+
+// ---------------------------------------------------------------- evaluation leaves
+template <int N>
+struct EvalLeaf {
+    const DevCtx *c;
+    const int *xr;
+    template <int I>
+    GRNS_HD void get(double &lo, double &up) {
+        double s = mulmod_recip_d((double)xr[I], c->dw[I], c->dm[I], c->dinv[I]);
+        lo = dmul_rd(s, c->rd[I]);
+        up = dmul_ru(s, c->ru[I]);
+    }
+};
+
+// Common path of rns_eval_compute[_fast] for one number held by this lane.  Returns false when
+// the number needs mixed-radix conversion or refinement (then `out` is not final).
+template <int N, bool FAST>
+GRNS_HD bool eval_hot(const DevCtx *c, const int (&xr)[N], interval_t &out) {
+    int nz = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) nz |= xr[i];
+    EvalLeaf<N> leaf{c, xr};
+    double suml, sumu;
+    psum_pair<N>(leaf, suml, sumu);
+    unsigned whl = to_uint(suml), whu = to_uint(sumu);
+    suml -= whl;
+    sumu -= whu;
+    out.low = er_set_d(suml);
+    out.upp = er_set_d(sumu);
+    if (nz == 0) {
+        out.low.frac = 0; out.low.exp = 0; out.upp.frac = 0; out.upp.exp = 0;
+        return true;
+    }
+    return (FAST || whl == whu) && (sumu >= c->accuracy);
+}
+
+GRNS_D void store_interval(interval_t *p, const interval_t &v) {
+    double2 *q = reinterpret_cast<double2 *>(p);
+    q[0] = make_double2(v.low.frac, __longlong_as_double(v.low.exp));
+    q[1] = make_double2(v.upp.frac, __longlong_as_double(v.upp.exp));
+}
+
+template <int N>
+GRNS_D void load_number(const int *__restrict__ x, long long smod, long long snum, long long j, int (&xr)[N]) {
+    const int *p = x + j * snum;
+#pragma unroll
+    for (int i = 0; i < N; i++) xr[i] = __builtin_nontemporal_load(p + (long long)i * smod);
+}
+
+// Resolve the lanes of `mask` with whole-group evaluations.
+template <bool FAST>
+GRNS_D void coop_eval_lanes(const DevCtx *c, int G, int L, const int *__restrict__ x, long long smod, long long snum,
+                            long long j, unsigned long long mask, interval_t *__restrict__ out, int *status) {
+    Group g = make_group(G, L);
+    const int P = 64 / G;
+    const int gid = (threadIdx.x & 63) / G;
+    while (mask) {
+        int src = kth_bit(mask, gid);
+        long long jl = __shfl(j, src < 0 ? 0 : src, 64);
+        mask = drop_low_bits(mask, P);
+        if (src >= 0) {
+            int xi = g.gl < L ? x[jl * snum + (long long)g.gl * smod] : 0;
+            int st = 0;
+            interval_t r = group_eval<FAST>(c, g, xi, st);
+            if (g.gl == 0) {
+                store_interval(out + jl, r);
+                if (st) atomicOr(status, st);
+            }
+        }
+    }
+}
+
+template <int N, bool FAST>
+__global__ __launch_bounds__(BLOCK) void k_eval(const DevCtx *__restrict__ c, const int *__restrict__ x, long long smod,
+                                                long long snum, long long count, interval_t *__restrict__ out,
+                                                int *__restrict__ status) {
+    for (long long base = (long long)blockIdx.x * BLOCK; base < count; base += (long long)gridDim.x * BLOCK) {
+        long long j = base + threadIdx.x;
+        bool rare = false;
+        if (j < count) {
+            int xr[N];
+            load_number<N>(x, smod, snum, j, xr);
+            interval_t r;
+            if (eval_hot<N, FAST>(c, xr, r)) store_interval(out + j, r);
+            else rare = true;
+        }
+        unsigned long long mask = __ballot(rare);
+        if (mask) coop_eval_lanes<FAST>(c, Pow2Ceil<N>::v, N, x, smod, snum, j, mask, out, status);
+    }
+}
+
+// Any n <= 64: one group per number throughout.
+template <bool FAST>
+__global__ __launch_bounds__(BLOCK) void k_eval_group(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
+                                                      long long smod, long long snum, long long count,
+                                                      interval_t *__restrict__ out, int *__restrict__ status) {
+    Group g = make_group(G, c->n);
+    const int P = 64 / G;
+    long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
+    for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
+        long long j = j0 + (threadIdx.x & 63) / G;
+        if (j < count) {
+            int xi = g.gl < g.L ? x[j * snum + (long long)g.gl * smod] : 0;
+            int st = 0;
+            interval_t r = group_eval<FAST>(c, g, xi, st);
+            if (g.gl == 0) {
+                store_interval(out + j, r);
+                if (st) atomicOr(status, st);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- filter sums
+// Approximation of x/M mod 1 from sum_i frac(x_i * w_i / m_i) with w_i/m_i held as a double-double;
+// |F - x/M mod 1| <= n^2 2^-52 (c->err bounds it generously).  `regular` is true when the
+// residues are canonical, every term frac(x_i w_i / m_i) lies in (2^-20, 1 - 2^-20) (so the CPU
+// path's reciprocal reduction of x_i w_i is canonical: its quotient error is below 2^-20.4) and
+// F lies in (tau, 1 - tau) (so its evaluation needs neither mixed-radix conversion nor
+// refinement and is a valid enclosure of x/M).
+GRNS_HD double fract_d(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fract(v);
+#else
+    return v - floor(v);
+#endif
+}
+
+template <int N>
+GRNS_HD double filter_frac(const DevCtx *c, const int (&xr)[N], bool &regular, bool &canon) {
+    double acc = 0.0, dev = 0.0;
+    bool cn = true;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double dx = (double)xr[i];
+        cn &= (unsigned)xr[i] < (unsigned)c->m[i];
+        double P = dx * c->chi[i];
+        double e = fma(dx, c->chi[i], -P);
+        double t = fma(dx, c->clo[i], e);
+        double fi = fract_d(P) + t;
+        dev = fmax(dev, fabs(fi - 0.5));
+        acc += fi;
+    }
+    double F = fract_d(acc);
+    canon = cn;
+    regular = cn && dev < 0.5 - 0x1p-20 && F > c->tau && F < 1.0 - c->tau;
+    return F;
+}
+
+// ---------------------------------------------------------------- comparison
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_cmp(const DevCtx *__restrict__ c, const int *__restrict__ x, long long xsm,
+                                               long long xsn, const int *__restrict__ y, long long ysm, long long ysn,
+                                               long long count, int *__restrict__ sign, int *__restrict__ status) {
+    for (long long base = (long long)blockIdx.x * BLOCK; base < count; base += (long long)gridDim.x * BLOCK) {
+        long long j = base + threadIdx.x;
+        bool rare = false;
+        if (j < count) {
+            int xr[N], yr[N];
+            load_number<N>(x, xsm, xsn, j, xr);
+            load_number<N>(y, ysm, ysn, j, yr);
+            bool rx, ry, cx, cy;
+            double fx = filter_frac<N>(c, xr, rx, cx);
+            double fy = filter_frac<N>(c, yr, ry, cy);
+            int res = 0;
+            if (rx && ry) {
+                // Both CPU-path evaluations are valid enclosures: rns_cmp returns the exact sign.
+                double e2 = 2.0 * c->err;
+                if (fx - fy > e2) res = 1;
+                else if (fy - fx > e2) res = -1;
+                else {
+                    bool eq = true;
+#pragma unroll
+                    for (int i = 0; i < N; i++) eq &= (xr[i] == yr[i]);
+                    if (!eq) rare = true;
+                }
+            } else {
+                rare = true;
+            }
+            if (!rare) sign[j] = res;
+        }
+        unsigned long long mask = __ballot(rare);
+        if (mask) {
+            const int G = Pow2Ceil<N>::v;
+            Group g = make_group(G, N);
+            const int P = 64 / G;
+            const int gid = (threadIdx.x & 63) / G;
+            while (mask) {
+                int src = kth_bit(mask, gid);
+                long long jl = __shfl(j, src < 0 ? 0 : src, 64);
+                mask = drop_low_bits(mask, P);
+                if (src >= 0) {
+                    int xi = g.gl < N ? x[jl * xsn + (long long)g.gl * xsm] : 0;
+                    int yi = g.gl < N ? y[jl * ysn + (long long)g.gl * ysm] : 0;
+                    int st = 0;
+                    int r = group_cmp(c, g, xi, yi, st);
+                    if (g.gl == 0) {
+                        sign[jl] = r;
+                        if (st) atomicOr(status, st);
+                    }
+                }
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_cmp_group(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
+                                                     long long xsm, long long xsn, const int *__restrict__ y,
+                                                     long long ysm, long long ysn, long long count,
+                                                     int *__restrict__ sign, int *__restrict__ status) {
+    Group g = make_group(G, c->n);
+    const int P = 64 / G;
+    long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
+    for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
+        long long j = j0 + (threadIdx.x & 63) / G;
+        if (j < count) {
+            int xi = g.gl < g.L ? x[j * xsn + (long long)g.gl * xsm] : 0;
+            int yi = g.gl < g.L ? y[j * ysn + (long long)g.gl * ysm] : 0;
+            int st = 0;
+            int r = group_cmp(c, g, xi, yi, st);
+            if (g.gl == 0) {
+                sign[j] = r;
+                if (st) atomicOr(status, st);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- arg-max (rns_max)
+// Candidates carry rigorous bounds [lo, hi] on x/M (or [-1, 2] when unknown); two candidates whose
+// bounds overlap are ordered exactly by mixed-radix digits, equal values by the lower index.
+struct MaxCand {
+    double lo;
+    double hi;
+    long long idx;
+};
+
+GRNS_D int canon_res(int v, int m, double dinv) {
+    long long r = rem_trunc((long long)v, m, dinv);
+    return (int)(r < 0 ? r + m : r);
+}
+
+// Exact comparison of numbers a and b of x by the group (value = residues taken mod m_i).
+GRNS_D int group_cmp_exact(const DevCtx *c, const Group &g, const int *__restrict__ x, long long smod, long long snum,
+                           long long a, long long b) {
+    int xa = 0, xb = 0;
+    if (g.gl < g.L) {
+        int mi = c->m[g.gl];
+        double dinv = c->dinv[g.gl];
+        xa = canon_res(x[a * snum + (long long)g.gl * smod], mi, dinv);
+        xb = canon_res(x[b * snum + (long long)g.gl * smod], mi, dinv);
+    }
+    if (group_all(g, g.gl >= g.L || xa == xb)) return 0;
+    int ma = group_mrc(c, g, xa);
+    int mb = group_mrc(c, g, xb);
+    return group_mrs_cmp(g, ma, mb);
+}
+
+// Decide whether `e` replaces `best`: 1 yes, 0 no, 2 needs the exact order.
+GRNS_D int max_decide(const MaxCand &e, const MaxCand &best) {
+    if (e.idx < 0) return 0;
+    if (best.idx < 0) return 1;
+    if (e.lo > best.hi) return 1;
+    if (best.lo > e.hi) return 0;
+    return 2;
+}
+
+// Every lane proposes (e -> best); pending lanes are resolved by the groups of the wave.  All
+// 64 lanes of the wave must call this.
+GRNS_D void max_merge(const DevCtx *c, int G, const int *__restrict__ x, long long smod, long long snum, MaxCand &best,
+                      const MaxCand &e, bool valid) {
+    int d = valid ? max_decide(e, best) : 0;
+    bool take = (d == 1);
+    unsigned long long mask = __ballot(d == 2);
+    if (mask) {
+        Group g = make_group(G, c->n);
+        const int P = 64 / G;
+        const int gid = (threadIdx.x & 63) / G;
+        const int lane = threadIdx.x & 63;
+        while (mask) {
+            int src = kth_bit(mask, gid);  // the same on every lane of a group
+            long long ea = __shfl(e.idx, src < 0 ? 0 : src, 64);
+            long long bb = __shfl(best.idx, src < 0 ? 0 : src, 64);
+            mask = drop_low_bits(mask, P);
+            int r = 0;
+            if (src >= 0) r = group_cmp_exact(c, g, x, smod, snum, ea, bb);
+            // hand each group's answer back to the lane that asked
+            for (int k = 0; k < P; k++) {
+                int s = __shfl(src, k * G, 64);
+                int ans = __shfl(r, k * G, 64);
+                if (s >= 0 && lane == s) take = (ans > 0) || (ans == 0 && e.idx < best.idx);
+            }
+        }
+    }
+    if (take) best = e;
+}
+
+template <int N>
+GRNS_D MaxCand max_key(const DevCtx *c, const int (&xr)[N], long long j) {
+    bool reg, canon;
+    double F = filter_frac<N>(c, xr, reg, canon);
+    MaxCand e;
+    e.idx = j;
+    double E = c->err;
+    if (canon && F > 2.0 * E && F < 1.0 - 2.0 * E) {
+        e.lo = F - E;
+        e.hi = F + E;
+    } else {
+        e.lo = -1.0;
+        e.hi = 2.0;
+    }
+    return e;
+}
+
+GRNS_D void block_max_reduce(const DevCtx *c, int G, const int *__restrict__ x, long long smod, long long snum,
+                             MaxCand &best, MaxCand *sh) {
+    sh[threadIdx.x] = best;
+    __syncthreads();
+    for (int h = BLOCK / 2; h >= 1; h >>= 1) {
+        MaxCand other = (threadIdx.x < (unsigned)h) ? sh[threadIdx.x + h] : best;
+        MaxCand mine = sh[threadIdx.x];
+        __syncthreads();
+        MaxCand b = mine;
+        max_merge(c, G, x, smod, snum, b, other, threadIdx.x < (unsigned)h);
+        if (threadIdx.x < (unsigned)h) sh[threadIdx.x] = b;
+        __syncthreads();
+    }
+    best = sh[0];
+}
+
+// Stage 1: each block reduces a contiguous chunk to one candidate.
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_max_partial(const DevCtx *__restrict__ c, const int *__restrict__ x,
+                                                       long long smod, long long snum, long long count,
+                                                       long long chunk, MaxCand *__restrict__ part) {
+    __shared__ MaxCand sh[BLOCK];
+    const int G = Pow2Ceil<N>::v;
+    long long start = (long long)blockIdx.x * chunk;
+    long long end = start + chunk < count ? start + chunk : count;
+    MaxCand best;
+    best.lo = -1.0; best.hi = -1.0; best.idx = -1;
+    for (long long b0 = start; b0 < end; b0 += BLOCK) {
+        long long j = b0 + threadIdx.x;
+        MaxCand e;
+        e.idx = -1; e.lo = -1.0; e.hi = -1.0;
+        bool valid = j < end;
+        if (valid) {
+            int xr[N];
+            load_number<N>(x, smod, snum, j, xr);
+            e = max_key<N>(c, xr, j);
+        }
+        max_merge(c, G, x, smod, snum, best, e, valid);
+    }
+    block_max_reduce(c, G, x, smod, snum, best, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = best;
+}
+
+// Stage 2: one block reduces the per-block candidates and writes the winner.
+__global__ __launch_bounds__(BLOCK) void k_max_final(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
+                                                     long long smod, long long snum, const MaxCand *__restrict__ part,
+                                                     int nparts, int *__restrict__ out, long long *__restrict__ out_idx) {
+    __shared__ MaxCand sh[BLOCK];
+    MaxCand best;
+    best.lo = -1.0; best.hi = -1.0; best.idx = -1;
+    for (int b0 = 0; b0 < nparts; b0 += BLOCK) {
+        int k = b0 + threadIdx.x;
+        MaxCand e;
+        e.idx = -1; e.lo = -1.0; e.hi = -1.0;
+        bool valid = k < nparts;
+        if (valid) e = part[k];
+        max_merge(c, G, x, smod, snum, best, e, valid && e.idx >= 0);
+    }
+    block_max_reduce(c, G, x, smod, snum, best, sh);
+    if (threadIdx.x < (unsigned)c->n && best.idx >= 0) out[threadIdx.x] = x[best.idx * snum + (long long)threadIdx.x * smod];
+    if (threadIdx.x == 0) *out_idx = best.idx;
+}
+
+// ---------------------------------------------------------------- division (group per number)
+// rns_div (FAST=false, src/rnsdiv.cuh:19-61) and rns_div_fast (FAST=true, :67-114).
+template <bool FAST>
+GRNS_D void group_div(const DevCtx *c, const Group &g, int xi, int di, int &q, int &r, int &status) {
+    const bool act = g.gl < g.L;
+    const int mi = act ? c->m[g.gl] : 1;
+    const double dinv = act ? c->dinv[g.gl] : 1.0;
+    const int n = c->n;
+    q = 0;
+    r = xi;
+    interval_t deval = group_eval<false>(c, g, di, status);
+    interval_t reval = group_eval<false>(c, g, r, status);
+    long long it = 0;
+    while (er_ucmp(reval.low, deval.upp) >= 0) {
+        er_t flq = er_div_rd(reval.low, deval.upp);
+        int pq = 0;
+        if (++it > 20000) { status |= GST_DIVLOOP; return; }
+        if (FAST) {
+            bool got = false;
+            while (flq.exp >= 0 && flq.frac > 0) {
+                if (flq.exp > c->log2M) { status |= GST_DIVLOOP; return; }
+                int p2 = act ? c->pow2[(size_t)flq.exp * n + g.gl] : 0;
+                pq = addmod_recip(pq, p2, mi, dinv);
+                flq.frac = dsub_rd(flq.frac, 1.0);
+                flq = er_adjust(flq);
+                got = true;
+            }
+            if (!got) { status |= GST_DIVLOOP; return; }
+        } else {
+            if (flq.exp < 0 || flq.exp > c->log2M) { status |= GST_DIVLOOP; return; }
+            pq = act ? c->pow2[(size_t)flq.exp * n + g.gl] : 0;
+        }
+        q = addmod_recip(q, pq, mi, dinv);
+        pq = mulmod_recip(pq, di, mi, dinv);
+        r = psubmod(r, pq, mi, dinv);
+        reval = group_eval<false>(c, g, r, status);
+    }
+    if (er_ucmp(reval.upp, deval.low) >= 0) {
+        int dmr = group_mrc(c, g, di);
+        int rmr = group_mrc(c, g, r);
+        // r >= d unless the most significant differing digit of r is smaller
+        bool rged = group_mrs_cmp(g, rmr, dmr) >= 0;
+        if (rged) {
+            q = addmod_recip(q, 1, mi, dinv);
+            r = psubmod(r, di, mi, dinv);
+        }
+    }
+}
+
+template <bool FAST>
+__global__ __launch_bounds__(BLOCK) void k_div(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
+                                               const int *__restrict__ d, long long smod, long long snum,
+                                               long long count, int *__restrict__ q, int *__restrict__ r,
+                                               const long long *__restrict__ order, int *__restrict__ status) {
+    Group g = make_group(G, c->n);
+    const int P = 64 / G;
+    long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
+    for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
+        long long jj = j0 + (threadIdx.x & 63) / G;
+        if (jj < count) {
+            long long j = order ? order[jj] : jj;
+            long long off = j * snum + (long long)g.gl * smod;
+            int xi = g.gl < g.L ? x[off] : 0;
+            int di = g.gl < g.L ? d[off] : 0;
+            int qq, rr, st = 0;
+            group_div<FAST>(c, g, xi, di, qq, rr, st);
+            if (g.gl < g.L) {
+                q[off] = qq;
+                r[off] = rr;
+            }
+            if (g.gl == 0 && st) atomicOr(status, st);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- scaling by 2^D (group per number)
+// rns_rank_compute / _fast, src/rnsscal.cuh:17-62
+GRNS_D int group_rank(const DevCtx *c, const Group &g, int xi, int s) {
+    const bool act = g.gl < g.L;
+    double fl = act ? dmul_rd((double)s, c->rd[g.gl]) : 0.0;
+    double fu = act ? dmul_ru((double)s, c->ru[g.gl]) : 0.0;
+    double suml = group_psum<false>(g, fl);
+    double sumu = group_psum<true>(g, fu);
+    unsigned whl = to_uint(suml), whu = to_uint(sumu);
+    if (whl == whu) return (int)whl;
+    int d = group_mrc(c, g, xi);
+    int mr = bcast_i(d, g.base + g.L - 1);
+    return mr == 0 ? (int)whu : (int)whl;
+}
+GRNS_D int group_rank_fast(const DevCtx *c, const Group &g, int s) {
+    const bool act = g.gl < g.L;
+    double fu = act ? dmul_ru((double)s, c->ru[g.gl]) : 0.0;
+    return (int)group_psum<true>(g, fu);
+}
+// make_scaling_step, src/rnsscal.cuh:67-85 (j in [1, 30], pow2j = 2^j)
+GRNS_D int group_scaling_step(const DevCtx *c, const Group &g, int k, int j, int xi, int ci) {
+    const bool act = g.gl < g.L;
+    const int mi = act ? c->m[g.gl] : 1;
+    const double dinv = act ? c->dinv[g.gl] : 1.0;
+    const long long pow2j = 1ll << j;
+    long long term = 0;
+    if (act) {
+        long long prod = (long long)c->part_pow2_res[j - 1][g.gl] * (long long)ci;
+        // C's truncating remainder by a power of two
+        term = prod >= 0 ? (prod & (pow2j - 1)) : -((-prod) & (pow2j - 1));
+    }
+    long long residue = group_sum_ll(g, term);
+    long long temp = (long long)k * (long long)c->prod_pow2_res[j - 1];
+    residue = (residue - temp) % pow2j;
+    if (residue < 0) residue += pow2j;
+    int multiple = act ? (int)(residue % mi) : 0;
+    multiple = act ? psubmod(xi, multiple, mi, dinv) : 0;
+    return act ? mulmod_recip(multiple, c->pow2_inv[j - 1][g.gl], mi, dinv) : 0;
+}
+// rns_scale2pow, src/rnsscal.cuh:93-119
+GRNS_D int group_scale2pow(const DevCtx *c, const Group &g, int xi, unsigned D) {
+    const bool act = g.gl < g.L;
+    const int mi = act ? c->m[g.gl] : 1;
+    const double dinv = act ? c->dinv[g.gl] : 1.0;
+    const int w = act ? c->w[g.gl] : 0;
+    int result = xi;
+    int t = (int)(D / THRESH);
+    int k;
+    if (t > 0) {
+        int ci = act ? mulmod_recip(xi, w, mi, dinv) : 0;
+        k = group_rank(c, g, xi, ci);
+        result = group_scaling_step(c, g, k, THRESH, xi, ci);
+        t -= 1;
+    }
+    while (t > 0) {
+        int ci = act ? mulmod_recip(result, w, mi, dinv) : 0;
+        k = group_rank_fast(c, g, ci);
+        result = group_scaling_step(c, g, k, THRESH, result, ci);
+        t -= 1;
+    }
+    unsigned d = D % THRESH;
+    if (d > 0) {
+        int ci = act ? mulmod_recip(result, w, mi, dinv) : 0;
+        k = d < D ? group_rank_fast(c, g, ci) : group_rank(c, g, result, ci);
+        result = group_scaling_step(c, g, k, (int)d, result, ci);
+    }
+    return result;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scale2pow(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
+                                                     long long smod, long long snum, const unsigned *__restrict__ D,
+                                                     long long count, int *__restrict__ y,
+                                                     const long long *__restrict__ order) {
+    Group g = make_group(G, c->n);
+    const int P = 64 / G;
+    long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
+    for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
+        long long jj = j0 + (threadIdx.x & 63) / G;
+        if (jj < count) {
+            long long j = order ? order[jj] : jj;
+            long long off = j * snum + (long long)g.gl * smod;
+            int xi = g.gl < g.L ? x[off] : 0;
+            int v = group_scale2pow(c, g, xi, D[j]);
+            if (g.gl < g.L) y[off] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- mpint add / sub / mul
+// mp_int_t split into arrays: digits (modulus-major), sign, eval[2].
+// mpint_check_overflow, src/mpint.cuh:142-152 -> 0 none, 1 overflow, 2 possible overflow
+GRNS_D unsigned char check_overflow(const DevCtx *c, const er_t &lower, const er_t &upper) {
+    if (upper.exp < -1 || er_ucmp(upper, c->inv_low) <= 0) return 0;
+    if (er_ucmp(lower, c->inv_upp) == 1) return 1;
+    return 2;
+}
+
+GRNS_D er_t load_er(const er_t *p) {
+    double2 v = *reinterpret_cast<const double2 *>(p);
+    er_t r;
+    r.frac = v.x;
+    r.exp = __double_as_longlong(v.y);
+    return r;
+}
+GRNS_D void store_er(er_t *p, const er_t &v) {
+    *reinterpret_cast<double2 *>(p) = make_double2(v.frac, __longlong_as_double(v.exp));
+}
+
+// OP: 0 add (src/mpint.cuh:160-207), 1 sub (:215-262), 2 mul (:270-278)
+template <int N, int OP>
+__global__ __launch_bounds__(BLOCK) void k_mpint(const DevCtx *__restrict__ c, long long ld, long long count,
+                                                 const int *__restrict__ xd, const int *__restrict__ xs,
+                                                 const er_t *__restrict__ xe, const int *__restrict__ yd,
+                                                 const int *__restrict__ ys, const er_t *__restrict__ ye,
+                                                 int *__restrict__ rd, int *__restrict__ rs, er_t *__restrict__ re,
+                                                 unsigned char *__restrict__ ovf) {
+    for (long long j = (long long)blockIdx.x * BLOCK + threadIdx.x; j < count; j += (long long)gridDim.x * BLOCK) {
+        int a[N], b[N], r[N];
+        load_number<N>(xd, ld, 1, j, a);
+        load_number<N>(yd, ld, 1, j, b);
+        int sx = xs[j], sy = ys[j];
+        er_t x0 = load_er(xe + 2 * j), x1 = load_er(xe + 2 * j + 1);
+        er_t y0 = load_er(ye + 2 * j), y1 = load_er(ye + 2 * j + 1);
+        er_t r0, r1;
+        int sr;
+        if constexpr (OP == 2) {
+            r0 = er_md_rd(x0, y0, c->unit_upp);
+            r1 = er_md_ru(x1, y1, c->unit_low);
+            int nz = 0;
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                r[i] = mulmod_recip(a[i], b[i], c->m[i], c->dinv[i]);
+                nz |= r[i];
+            }
+            sr = (nz == 0) ? 0 : (sx ^ sy);
+        } else {
+            if (OP == 1) sy ^= 1;
+            int alpha = 1 - 2 * sx, beta = 1 - 2 * sy;
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                int res = addmod_recip(alpha * a[i], beta * b[i], c->m[i], c->dinv[i]);
+                r[i] = res < 0 ? res + c->m[i] : res;
+            }
+            x0.frac *= alpha; x1.frac *= alpha;
+            y0.frac *= beta; y1.frac *= beta;
+            er_t ea = sx ? x1 : x0, eb = sy ? y1 : y0;
+            er_t fa = sx ? x0 : x1, fb = sy ? y0 : y1;
+            r0 = er_addsub<false, false>(ea, eb);
+            r1 = er_addsub<false, true>(fa, fb);
+            int minus = r0.frac < 0 && r1.frac < 0;
+            sr = minus;
+            if (minus) {
+#pragma unroll
+                for (int i = 0; i < N; i++) r[i] = r[i] == 0 ? 0 : (c->m[i] - r[i]);
+                er_t tmp = r0;
+                r0.frac = -1 * r1.frac;
+                r0.exp = r1.exp;
+                r1.frac = -1 * tmp.frac;
+                r1.exp = tmp.exp;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < N; i++) __builtin_nontemporal_store(r[i], rd + (long long)i * ld + j);
+        rs[j] = sr;
+        store_er(re + 2 * j, r0);
+        store_er(re + 2 * j + 1, r1);
+        if (ovf) ovf[j] = check_overflow(c, r0, r1);
+    }
+}
+
+}  // namespace grns

@@ -1,0 +1,545 @@
+// grns_lib.hip -- host side of libgrns_hip.so: moduli-set contexts (the constants of
+// rns_const_init, src/rnsbase.cuh:324-451, computed without GMP/MPFR), kernel launchers and the
+// C ABI declared in include/grns.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/grns.h"
+#include "../../include/grns_debug.h"
+#include "grns_kernels.hpp"
+
+using namespace grns;
+
+struct grns_ctx {
+    int device = 0;
+    int n = 0;
+    int log2M = 0;
+    DevCtx h;                      // host copy of the device constants
+    DevCtx *d = nullptr;           // device constants
+    int *d_pow2 = nullptr;
+    int *d_mrc = nullptr;
+    int *d_status = nullptr;
+    std::vector<int> pow2;
+    std::vector<int> mrc;
+};
+
+namespace {
+
+#define HIP_OK(call)                                   \
+    do {                                               \
+        if ((call) != hipSuccess) return GRNS_ERR_HIP; \
+    } while (0)
+
+// Canonical inverse of x modulo m (extended Euclid); m >= 2, gcd(x, m) = 1.
+int inverse_mod(long long x, long long m) {
+    long long a = ((x % m) + m) % m, b = m, u = 1, v = 0;
+    while (b != 0) {
+        long long q = a / b, t = a - q * b;
+        a = b; b = t;
+        t = u - q * v; u = v; v = t;
+    }
+    return (int)(((u % m) + m) % m);
+}
+
+// Round-to-nearest directed bounds exactly as the reference's host code forms them
+// (src/dinterval.cuh:350-362, unfused: this file is compiled with -ffp-contract=off).
+double host_ddiv_rd(double a, double b) { double c = a / b; return c - (PHI1 * std::fabs(c) + ETA); }
+double host_ddiv_ru(double a, double b) { double c = a / b; return c + (PHI1 * std::fabs(c) + ETA); }
+
+// frac in [0.5, 1) (the mpfr_get_d_2exp convention of src/rnsbase.cuh:399-409) from a significand
+// num in (2^52, 2^53] scaled by 2^-53.
+er_t half_er(unsigned long long num, long long exp) {
+    er_t r;
+    if (num == (1ull << 53)) { r.frac = 0.5; r.exp = exp + 1; }
+    else { r.frac = std::ldexp((double)num, -53); r.exp = exp; }
+    return r;
+}
+
+grns_status build_constants(grns_ctx *c, const int *moduli, int n) {
+    DevCtx &h = c->h;
+    std::memset(&h, 0, sizeof(h));
+    h.n = n;
+    // M as base-2^32 limbs
+    std::vector<unsigned> limbs(1, 1u);
+    for (int i = 0; i < n; i++) {
+        unsigned long long carry = 0;
+        for (auto &l : limbs) {
+            unsigned long long t = (unsigned long long)l * (unsigned)moduli[i] + carry;
+            l = (unsigned)t;
+            carry = t >> 32;
+        }
+        if (carry) limbs.push_back((unsigned)carry);
+    }
+    int bitlen = (int)(limbs.size() - 1) * 32 + (32 - __builtin_clz(limbs.back()));
+    h.log2M = bitlen - 1;
+    c->log2M = h.log2M;
+    h.M64 = (unsigned long long)limbs[0] | (limbs.size() > 1 ? (unsigned long long)limbs[1] << 32 : 0ull);
+
+    for (int i = 0; i < n; i++) {
+        const long long m = moduli[i];
+        h.m[i] = moduli[i];
+        long long p = 1 % m;
+        unsigned long long p64 = 1;
+        for (int j = 0; j < n; j++) {
+            if (j == i) continue;
+            p = (p * (moduli[j] % m)) % m;
+            p64 *= (unsigned long long)moduli[j];
+        }
+        h.w[i] = inverse_mod(p, m);        // (M/m_i)^-1 mod m_i
+        h.mi64[i] = p64;
+        h.dm[i] = (double)moduli[i];
+        h.dw[i] = (double)h.w[i];
+        h.dinv[i] = 1.0 / (double)moduli[i];
+        h.rd[i] = host_ddiv_rd(1.0, (double)moduli[i]);
+        h.ru[i] = host_ddiv_ru(1.0, (double)moduli[i]);
+        h.chi[i] = (double)h.w[i] / (double)moduli[i];
+        double rem = std::fma(-h.chi[i], (double)moduli[i], (double)h.w[i]);  // exact remainder
+        h.clo[i] = rem / (double)moduli[i];
+        h.c32[i] = (int)((1ull << 32) % (unsigned long long)m);
+    }
+    for (int t = 0; t < THRESH; t++) {
+        unsigned long long mask = (1ull << (t + 1)) - 1;
+        unsigned long long pm = 1;
+        for (int i = 0; i < n; i++) pm = (pm * (unsigned long long)moduli[i]) & mask;
+        h.prod_pow2_res[t] = (int)pm;
+        for (int j = 0; j < n; j++) {
+            unsigned long long pp = 1;
+            for (int i = 0; i < n; i++)
+                if (i != j) pp = (pp * (unsigned long long)moduli[i]) & mask;
+            h.part_pow2_res[t][j] = (int)pp;
+            long long pw = 1;
+            for (int k = 0; k <= t; k++) pw = (pw * 2) % moduli[j];
+            h.pow2_inv[t][j] = inverse_mod(pw, moduli[j]);
+        }
+    }
+    // src/rnsbase.cuh:389-391 (same expression, same libm)
+    h.accuracy = 4 * std::pow(2.0, 1 - 53) * n * std::log2((double)n) * (1 + 0.0000001 / 2) / 0.0000001;
+    h.ref_factor = (int)std::floor(std::log2(1 / (2 * h.accuracy)));
+    // Filter windows: the evaluation takes its common path when x/M lies in (tau, 1 - tau);
+    // tau is a power of two at least 4x the accuracy constant.
+    double tau = 0x1p-12;
+    while (tau < 4 * h.accuracy) tau *= 2;
+    h.tau = tau;
+    h.err = 0x1p-30;
+    // RNS_EVAL_UNIT / INV_UNIT: M rounded down to 53 bits, then 1/Md and 1 - 1/Md rounded up and
+    // down to 53 bits in the [0.5, 1) convention.
+    {
+        int e = bitlen > 53 ? bitlen - 53 : 0;
+        unsigned __int128 acc = 0;
+        int w0 = e / 32, sh = e % 32;
+        for (int k = 3; k >= 0; k--) {
+            size_t w = (size_t)(w0 + k);
+            acc = (acc << 32) | (unsigned __int128)(w < limbs.size() ? limbs[w] : 0u);
+        }
+        unsigned long long a = (unsigned long long)((acc >> sh) & (((unsigned __int128)1 << 53) - 1));
+        int b = 64 - __builtin_clzll(a);
+        unsigned __int128 num = (unsigned __int128)1 << (b - 1 + 53);
+        unsigned long long q = (unsigned long long)(num / a), r = (unsigned long long)(num % a);
+        long long ex = 1 - b - e;
+        h.unit_low = half_er(q, ex);
+        h.unit_upp = half_er(q + (r != 0), ex);
+        const unsigned long long two53 = 1ull << 53;
+        if (bitlen <= 53) {
+            unsigned __int128 nn = (unsigned __int128)(a - 1) << 53;
+            unsigned long long qd = (unsigned long long)(nn / a), rd = (unsigned long long)(nn % a);
+            h.inv_low = half_er(qd, 0);
+            h.inv_upp = half_er(qd + (rd != 0), 0);
+        } else {
+            bool exact = (e == 1 && a == (1ull << 52));
+            h.inv_low = half_er(two53 - 1, 0);
+            h.inv_upp = half_er(exact ? two53 - 1 : two53, 0);
+        }
+    }
+    // RNS_POW2 and MRC_MULT_INV
+    c->pow2.assign((size_t)(h.log2M + 1) * n, 0);
+    for (int j = 0; j < n; j++) {
+        long long v = 1;
+        for (int i = 0; i <= h.log2M; i++) {
+            c->pow2[(size_t)i * n + j] = (int)v;
+            v = (v * 2) % moduli[j];
+        }
+    }
+    c->mrc.assign((size_t)n * n, 0);
+    for (int i = 0; i < n; i++)
+        for (int j = i + 1; j < n; j++) c->mrc[(size_t)i * n + j] = inverse_mod(moduli[i], moduli[j]);
+    return GRNS_OK;
+}
+
+int groups_for(int n) {
+    int G = 1;
+    while (G < n) G <<= 1;
+    return G;
+}
+
+hipStream_t S(void *s) { return (hipStream_t)s; }
+
+long long grid_elems(size_t count) {
+    long long b = (long long)((count + BLOCK - 1) / BLOCK);
+    if (b < 1) b = 1;
+    if (b > (1ll << 20)) b = 1ll << 20;
+    return b;
+}
+long long grid_groups(size_t count, int G) {
+    long long per_wave = 64 / G;
+    long long waves = (long long)((count + per_wave - 1) / per_wave);
+    long long b = (waves + (BLOCK / 64) - 1) / (BLOCK / 64);
+    if (b < 1) b = 1;
+    if (b > (1ll << 20)) b = 1ll << 20;
+    return b;
+}
+
+grns_status launch_check() { return hipGetLastError() == hipSuccess ? GRNS_OK : GRNS_ERR_HIP; }
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char *grns_version(void) { return "grns_amd 0.1 (gfx950)"; }
+
+grns_status grns_ctx_create(const int *moduli, int n, int device, grns_ctx **out) {
+    if (!moduli || !out || n < 1 || n > MAXN) return GRNS_ERR_ARG;
+    for (int i = 0; i < n; i++)
+        if (moduli[i] < 2) return GRNS_ERR_ARG;
+    grns_ctx *c = new (std::nothrow) grns_ctx();
+    if (!c) return GRNS_ERR_OOM;
+    c->device = device;
+    c->n = n;
+    grns_status st = build_constants(c, moduli, n);
+    if (st != GRNS_OK) { delete c; return st; }
+    if (device < 0) {  // host-only context: constants and host debug entry points, no device memory
+        *out = c;
+        return GRNS_OK;
+    }
+    DeviceGuard guard(device);
+    if (hipMalloc(&c->d, sizeof(DevCtx)) != hipSuccess ||
+        hipMalloc(&c->d_pow2, sizeof(int) * c->pow2.size()) != hipSuccess ||
+        hipMalloc(&c->d_mrc, sizeof(int) * c->mrc.size()) != hipSuccess ||
+        hipMalloc(&c->d_status, sizeof(int)) != hipSuccess) {
+        grns_ctx_destroy(c);
+        return GRNS_ERR_OOM;
+    }
+    c->h.pow2 = c->d_pow2;
+    c->h.mrc_inv = c->d_mrc;
+    if (hipMemcpy(c->d_pow2, c->pow2.data(), sizeof(int) * c->pow2.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_mrc, c->mrc.data(), sizeof(int) * c->mrc.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d, &c->h, sizeof(DevCtx), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->d_status, 0, sizeof(int)) != hipSuccess) {
+        grns_ctx_destroy(c);
+        return GRNS_ERR_HIP;
+    }
+    *out = c;
+    return GRNS_OK;
+}
+
+grns_status grns_ctx_destroy(grns_ctx *c) {
+    if (!c) return GRNS_ERR_ARG;
+    if (c->device >= 0) {
+        DeviceGuard guard(c->device);
+        if (c->d) (void)hipFree(c->d);
+        if (c->d_pow2) (void)hipFree(c->d_pow2);
+        if (c->d_mrc) (void)hipFree(c->d_mrc);
+        if (c->d_status) (void)hipFree(c->d_status);
+    }
+    delete c;
+    return GRNS_OK;
+}
+
+int grns_ctx_n(const grns_ctx *c) { return c ? c->n : 0; }
+int grns_ctx_log2m(const grns_ctx *c) { return c ? c->log2M : 0; }
+
+grns_status grns_ctx_constants(const grns_ctx *c, int *part_inv, double *recip_rd, double *recip_ru,
+                               double *accuracy, int *ref_factor, grns_interval *unit, grns_interval *inv_unit) {
+    if (!c) return GRNS_ERR_ARG;
+    for (int i = 0; i < c->n; i++) {
+        if (part_inv) part_inv[i] = c->h.w[i];
+        if (recip_rd) recip_rd[i] = c->h.rd[i];
+        if (recip_ru) recip_ru[i] = c->h.ru[i];
+    }
+    if (accuracy) *accuracy = c->h.accuracy;
+    if (ref_factor) *ref_factor = c->h.ref_factor;
+    if (unit) {
+        unit->low.frac = c->h.unit_low.frac; unit->low.exp = c->h.unit_low.exp;
+        unit->upp.frac = c->h.unit_upp.frac; unit->upp.exp = c->h.unit_upp.exp;
+    }
+    if (inv_unit) {
+        inv_unit->low.frac = c->h.inv_low.frac; inv_unit->low.exp = c->h.inv_low.exp;
+        inv_unit->upp.frac = c->h.inv_upp.frac; inv_unit->upp.exp = c->h.inv_upp.exp;
+    }
+    return GRNS_OK;
+}
+
+grns_status grns_ctx_tables(const grns_ctx *c, int *pow2, int *mrc_inv) {
+    if (!c) return GRNS_ERR_ARG;
+    if (pow2) std::memcpy(pow2, c->pow2.data(), sizeof(int) * c->pow2.size());
+    if (mrc_inv) std::memcpy(mrc_inv, c->mrc.data(), sizeof(int) * c->mrc.size());
+    return GRNS_OK;
+}
+
+grns_status grns_ctx_status(grns_ctx *c, int *bits) {
+    if (!c || !bits || c->device < 0) return GRNS_ERR_ARG;
+    DeviceGuard guard(c->device);
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(bits, c->d_status, sizeof(int), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemset(c->d_status, 0, sizeof(int)));
+    return GRNS_OK;
+}
+
+grns_status grns_eval(grns_ctx *c, const int *x, long long smod, long long snum, size_t count, grns_interval *out,
+                      int fast, void *stream) {
+    if (c && c->device < 0) return GRNS_ERR_ARG;
+    if (!c || (!x && count) || (!out && count)) return GRNS_ERR_ARG;
+    if (count == 0) return GRNS_OK;
+    DeviceGuard guard(c->device);
+    interval_t *o = reinterpret_cast<interval_t *>(out);
+    dim3 grid((unsigned)grid_elems(count)), blk(BLOCK);
+    long long cnt = (long long)count;
+#define GRNS_EVAL_CASE(NN)                                                                                       \
+    case NN:                                                                                                     \
+        if (fast) hipLaunchKernelGGL((k_eval<NN, true>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, o, c->d_status); \
+        else hipLaunchKernelGGL((k_eval<NN, false>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, o, c->d_status);     \
+        return launch_check();
+    switch (c->n) {
+        GRNS_EVAL_CASE(4)
+        GRNS_EVAL_CASE(8)
+        GRNS_EVAL_CASE(16)
+        GRNS_EVAL_CASE(32)
+        GRNS_EVAL_CASE(64)
+        default:
+            break;
+    }
+#undef GRNS_EVAL_CASE
+    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
+    int G = groups_for(c->n);
+    dim3 g2((unsigned)grid_groups(count, G));
+    if (fast) hipLaunchKernelGGL((k_eval_group<true>), g2, blk, 0, S(stream), c->d, G, x, smod, snum, cnt, o, c->d_status);
+    else hipLaunchKernelGGL((k_eval_group<false>), g2, blk, 0, S(stream), c->d, G, x, smod, snum, cnt, o, c->d_status);
+    return launch_check();
+}
+
+grns_status grns_cmp(grns_ctx *c, const int *x, long long xsm, long long xsn, const int *y, long long ysm,
+                     long long ysn, size_t count, int *sign, void *stream) {
+    if (c && c->device < 0) return GRNS_ERR_ARG;
+    if (!c || (count && (!x || !y || !sign))) return GRNS_ERR_ARG;
+    if (count == 0) return GRNS_OK;
+    DeviceGuard guard(c->device);
+    dim3 grid((unsigned)grid_elems(count)), blk(BLOCK);
+    long long cnt = (long long)count;
+#define GRNS_CMP_CASE(NN)                                                                                          \
+    case NN:                                                                                                       \
+        hipLaunchKernelGGL((k_cmp<NN>), grid, blk, 0, S(stream), c->d, x, xsm, xsn, y, ysm, ysn, cnt, sign, c->d_status); \
+        return launch_check();
+    switch (c->n) {
+        GRNS_CMP_CASE(4)
+        GRNS_CMP_CASE(8)
+        GRNS_CMP_CASE(16)
+        GRNS_CMP_CASE(32)
+        GRNS_CMP_CASE(64)
+        default:
+            break;
+    }
+#undef GRNS_CMP_CASE
+    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
+    int G = groups_for(c->n);
+    dim3 g2((unsigned)grid_groups(count, G));
+    hipLaunchKernelGGL(k_cmp_group, g2, blk, 0, S(stream), c->d, G, x, xsm, xsn, y, ysm, ysn, cnt, sign, c->d_status);
+    return launch_check();
+}
+
+grns_status grns_div(grns_ctx *c, const int *x, const int *d, long long smod, long long snum, size_t count, int *q,
+                     int *r, int fast, void *stream) {
+    if (c && c->device < 0) return GRNS_ERR_ARG;
+    if (!c || (count && (!x || !d || !q || !r))) return GRNS_ERR_ARG;
+    if (count == 0) return GRNS_OK;
+    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
+    DeviceGuard guard(c->device);
+    int G = groups_for(c->n);
+    dim3 grid((unsigned)grid_groups(count, G)), blk(BLOCK);
+    long long cnt = (long long)count;
+    if (fast) hipLaunchKernelGGL((k_div<true>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, (const long long *)nullptr, c->d_status);
+    else hipLaunchKernelGGL((k_div<false>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, (const long long *)nullptr, c->d_status);
+    return launch_check();
+}
+
+grns_status grns_scale2pow(grns_ctx *c, const int *x, long long smod, long long snum, const unsigned *D, size_t count,
+                           int *y, void *stream) {
+    if (c && c->device < 0) return GRNS_ERR_ARG;
+    if (!c || (count && (!x || !D || !y))) return GRNS_ERR_ARG;
+    if (count == 0) return GRNS_OK;
+    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
+    DeviceGuard guard(c->device);
+    int G = groups_for(c->n);
+    dim3 grid((unsigned)grid_groups(count, G)), blk(BLOCK);
+    hipLaunchKernelGGL(k_scale2pow, grid, blk, 0, S(stream), c->d, G, x, smod, snum, D, (long long)count, y,
+                       (const long long *)nullptr);
+    return launch_check();
+}
+
+static void max_shape(size_t count, long long &nblocks, long long &chunk) {
+    const long long target = 2048;
+    long long per = (long long)((count + target - 1) / target);
+    per = ((per + BLOCK - 1) / BLOCK) * BLOCK;
+    if (per < BLOCK) per = BLOCK;
+    chunk = per;
+    nblocks = (long long)((count + per - 1) / per);
+    if (nblocks < 1) nblocks = 1;
+}
+
+size_t grns_max_workspace_size(const grns_ctx *c, size_t count) {
+    (void)c;
+    long long nb, ch;
+    max_shape(count, nb, ch);
+    return (size_t)nb * sizeof(MaxCand);
+}
+
+grns_status grns_max(grns_ctx *c, const int *x, long long smod, long long snum, size_t count, int *out_residues,
+                     long long *out_idx, void *workspace, size_t ws_bytes, void *stream) {
+    if (c && c->device < 0) return GRNS_ERR_ARG;
+    if (!c || !x || !out_residues || !out_idx || count == 0) return GRNS_ERR_ARG;
+    long long nb, ch;
+    max_shape(count, nb, ch);
+    if (!workspace || ws_bytes < (size_t)nb * sizeof(MaxCand)) return GRNS_ERR_ARG;
+    DeviceGuard guard(c->device);
+    MaxCand *part = reinterpret_cast<MaxCand *>(workspace);
+    dim3 grid((unsigned)nb), blk(BLOCK);
+    long long cnt = (long long)count;
+    int G = groups_for(c->n);
+#define GRNS_MAX_CASE(NN)                                                                                    \
+    case NN:                                                                                                 \
+        hipLaunchKernelGGL((k_max_partial<NN>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, ch, part); \
+        break;
+    switch (c->n) {
+        GRNS_MAX_CASE(4)
+        GRNS_MAX_CASE(8)
+        GRNS_MAX_CASE(16)
+        GRNS_MAX_CASE(32)
+        GRNS_MAX_CASE(64)
+        default:
+            return GRNS_ERR_UNSUPPORTED;
+    }
+#undef GRNS_MAX_CASE
+    if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;
+    hipLaunchKernelGGL(k_max_final, dim3(1), blk, 0, S(stream), c->d, G, x, smod, snum, part, (int)nb, out_residues,
+                       out_idx);
+    return launch_check();
+}
+
+static grns_status mpint_launch(grns_ctx *c, int op, const grns_mpint_array *x, const grns_mpint_array *y,
+                                const grns_mpint_array *z, size_t count, unsigned char *ovf, void *stream) {
+    if (!c || !x || !y || !z || c->device < 0) return GRNS_ERR_ARG;
+    if (count == 0) return GRNS_OK;
+    if (x->ld != y->ld || x->ld != z->ld || x->ld < (long long)count) return GRNS_ERR_ARG;
+    DeviceGuard guard(c->device);
+    dim3 grid((unsigned)grid_elems(count)), blk(BLOCK);
+    long long cnt = (long long)count;
+    const er_t *xe = reinterpret_cast<const er_t *>(x->eval);
+    const er_t *ye = reinterpret_cast<const er_t *>(y->eval);
+    er_t *ze = reinterpret_cast<er_t *>(z->eval);
+#define GRNS_MP_CASE(NN)                                                                                                  \
+    case NN:                                                                                                              \
+        if (op == 0) hipLaunchKernelGGL((k_mpint<NN, 0>), grid, blk, 0, S(stream), c->d, x->ld, cnt, x->digits, x->sign, xe, y->digits, y->sign, ye, z->digits, z->sign, ze, ovf); \
+        else if (op == 1) hipLaunchKernelGGL((k_mpint<NN, 1>), grid, blk, 0, S(stream), c->d, x->ld, cnt, x->digits, x->sign, xe, y->digits, y->sign, ye, z->digits, z->sign, ze, ovf); \
+        else hipLaunchKernelGGL((k_mpint<NN, 2>), grid, blk, 0, S(stream), c->d, x->ld, cnt, x->digits, x->sign, xe, y->digits, y->sign, ye, z->digits, z->sign, ze, ovf); \
+        return launch_check();
+    switch (c->n) {
+        GRNS_MP_CASE(4)
+        GRNS_MP_CASE(8)
+        GRNS_MP_CASE(16)
+        GRNS_MP_CASE(32)
+        GRNS_MP_CASE(64)
+        default:
+            return GRNS_ERR_UNSUPPORTED;
+    }
+#undef GRNS_MP_CASE
+}
+
+grns_status grns_mpint_add(grns_ctx *c, const grns_mpint_array *x, const grns_mpint_array *y,
+                           const grns_mpint_array *z, size_t count, unsigned char *ovf, void *stream) {
+    return mpint_launch(c, 0, x, y, z, count, ovf, stream);
+}
+grns_status grns_mpint_sub(grns_ctx *c, const grns_mpint_array *x, const grns_mpint_array *y,
+                           const grns_mpint_array *z, size_t count, unsigned char *ovf, void *stream) {
+    return mpint_launch(c, 1, x, y, z, count, ovf, stream);
+}
+grns_status grns_mpint_mul(grns_ctx *c, const grns_mpint_array *x, const grns_mpint_array *y,
+                           const grns_mpint_array *z, size_t count, unsigned char *ovf, void *stream) {
+    return mpint_launch(c, 2, x, y, z, count, ovf, stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- host debug entry points
+// The hot-path routines are __host__ __device__; these run the very same code on the CPU so that
+// it can be checked against the CPU restatement without a GPU.
+namespace {
+template <int N>
+int host_eval_hot_n(const grns_ctx *c, const int *x, grns_interval *out, int fast) {
+    int xr[N];
+    for (int i = 0; i < N; i++) xr[i] = x[i];
+    interval_t r;
+    bool ok = fast ? eval_hot<N, true>(&c->h, xr, r) : eval_hot<N, false>(&c->h, xr, r);
+    out->low.frac = r.low.frac; out->low.exp = r.low.exp;
+    out->upp.frac = r.upp.frac; out->upp.exp = r.upp.exp;
+    return ok ? 1 : 0;
+}
+template <int N>
+double host_filter_n(const grns_ctx *c, const int *x, int *regular, int *canon) {
+    int xr[N];
+    for (int i = 0; i < N; i++) xr[i] = x[i];
+    bool rg, cn;
+    double F = filter_frac<N>(&c->h, xr, rg, cn);
+    *regular = rg;
+    *canon = cn;
+    return F;
+}
+}  // namespace
+
+extern "C" {
+/* Common path of the evaluation for one number (residues x[0..n-1]); returns 1 when the
+ * common path is final, 0 when the number needs the cooperative path, -1 if n is unsupported. */
+int grns_debug_eval_hot(const grns_ctx *c, const int *x, grns_interval *out, int fast) {
+    switch (c->n) {
+        case 4: return host_eval_hot_n<4>(c, x, out, fast);
+        case 8: return host_eval_hot_n<8>(c, x, out, fast);
+        case 16: return host_eval_hot_n<16>(c, x, out, fast);
+        case 32: return host_eval_hot_n<32>(c, x, out, fast);
+        case 64: return host_eval_hot_n<64>(c, x, out, fast);
+        default: return -1;
+    }
+}
+double grns_debug_filter(const grns_ctx *c, const int *x, int *regular, int *canon) {
+    switch (c->n) {
+        case 4: return host_filter_n<4>(c, x, regular, canon);
+        case 8: return host_filter_n<8>(c, x, regular, canon);
+        case 16: return host_filter_n<16>(c, x, regular, canon);
+        case 32: return host_filter_n<32>(c, x, regular, canon);
+        case 64: return host_filter_n<64>(c, x, regular, canon);
+        default: *regular = 0; *canon = 0; return -1.0;
+    }
+}
+int grns_debug_ceil_log2(double s) { return ceil_log2_libm(s); }
+double grns_debug_mulmod_recip_d(double x, double y, double m, double d) { return mulmod_recip_d(x, y, m, d); }
+int grns_debug_mulmod_recip(int x, int y, int m, double d) { return mulmod_recip(x, y, m, d); }
+int grns_debug_addmod_recip(int x, int y, int m, double d) { return addmod_recip(x, y, m, d); }
+double grns_debug_bn_err(double c) { return bn_err(c); }
+double grns_debug_fast_scalbn(double x, int n) { return fast_scalbn(x, n); }
+double grns_debug_filter_tau(const grns_ctx *c) { return c->h.tau; }
+double grns_debug_filter_err(const grns_ctx *c) { return c->h.err; }
+}

@@ -1,0 +1,119 @@
+/*
+ * grns.h -- C ABI of libgrns_hip.so, the MI355X (gfx950) implementation of the GRNS
+ * data-parallel RNS hot path.
+ *
+ * Plain C: pointers, sizes and status codes.  Device pointers are HIP device memory owned by
+ * the caller; `stream` is a hipStream_t passed as void* (NULL = the default stream).  Calls are
+ * asynchronous on the stream unless stated otherwise.
+ *
+ * Residue arrays are described by a base pointer and two strides (in int32 elements): the
+ * residue of number j modulo m_i is at base[i * mod_stride + j * num_stride].  Modulus-major
+ * (mod_stride = count or more, num_stride = 1) is the layout the kernels are tuned for;
+ * number-major (the reference's [x1..xn][x1..xn]... layout: mod_stride = 1, num_stride = n) is
+ * accepted by the same kernels.
+ *
+ * Results are bit-identical to the CPU path of GRNS 1.1.4 (the global-namespace host functions
+ * named in each comment) on the same inputs.
+ */
+#ifndef GRNS_H
+#define GRNS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    GRNS_OK = 0,
+    GRNS_ERR_ARG = 1,          /* bad argument (null pointer, size, moduli set)            */
+    GRNS_ERR_HIP = 2,          /* a HIP runtime call failed                               */
+    GRNS_ERR_OOM = 3,          /* device allocation failed                                */
+    GRNS_ERR_UNSUPPORTED = 4,  /* moduli-set size not supported by this entry point       */
+    GRNS_ERR_ADMISSION = 5,    /* serving: request exceeds the HBM budget                 */
+    GRNS_ERR_BUSY = 6          /* serving: queue full, retry later                        */
+} grns_status;
+
+/* er_float_t (src/types.cuh:13-16): value = frac * 2^exp */
+typedef struct { double frac; long long exp; } grns_er;
+/* interval_t (src/types.cuh:22-25) */
+typedef struct { grns_er low; grns_er upp; } grns_interval;
+
+typedef struct grns_ctx grns_ctx;
+
+/* Device status bits raised by kernels for inputs on which the reference itself reads outside
+ * its tables or does not terminate (grns_ctx_status). */
+#define GRNS_STATUS_REFINE_RANGE 1   /* refinement factor outside RNS_POW2 (rnseval.cuh:82-83) */
+#define GRNS_STATUS_DIV_LOOP     2   /* division loop would not terminate (rnsdiv.cuh:34-41, 82-94) */
+
+/* ---------------------------------------------------------------- contexts
+ * Replaces rns_const_init() / mpint_const_init() (src/rnsbase.cuh:324-451, src/mpint.cuh:20-32):
+ * every constant table of one moduli set, resident on `device`.  n in [1, 256]; moduli >= 2,
+ * pairwise coprime, ascending (the params.h contract, src/params.h:11-39).  Synchronous. */
+grns_status grns_ctx_create(const int *moduli, int n, int device, grns_ctx **out);
+grns_status grns_ctx_destroy(grns_ctx *ctx);
+int grns_ctx_n(const grns_ctx *ctx);
+int grns_ctx_log2m(const grns_ctx *ctx);
+/* Host copies of the constants for inspection: part_inv[n], recip_rd[n], recip_ru[n],
+ * accuracy, ref_factor, unit/inv_unit (RNS_EVAL_UNIT / RNS_EVAL_INV_UNIT). Any pointer may be NULL. */
+grns_status grns_ctx_constants(const grns_ctx *ctx, int *part_inv, double *recip_rd, double *recip_ru,
+                               double *accuracy, int *ref_factor, grns_interval *unit, grns_interval *inv_unit);
+/* RNS_POW2 row-major ((log2m+1) x n) and MRC_MULT_INV (n x n) host copies. */
+grns_status grns_ctx_tables(const grns_ctx *ctx, int *pow2, int *mrc_inv);
+/* Synchronise the device and return (and clear) the status bits raised since the last call. */
+grns_status grns_ctx_status(grns_ctx *ctx, int *bits);
+
+/* ---------------------------------------------------------------- element-wise operations */
+
+/* rns_eval_compute (fast = 0, src/rnseval.cuh:29-102) or rns_eval_compute_fast (fast = 1,
+ * :112-167) of `count` numbers; out[j] is the interval of number j. */
+grns_status grns_eval(grns_ctx *ctx, const int *x, long long mod_stride, long long num_stride, size_t count,
+                      grns_interval *out, int fast, void *stream);
+
+/* rns_cmp (src/rnscmp.cuh:20-39): sign[j] = 1, 0, -1 for x_j >, =, < y_j. */
+grns_status grns_cmp(grns_ctx *ctx, const int *x, long long x_mod_stride, long long x_num_stride, const int *y,
+                     long long y_mod_stride, long long y_num_stride, size_t count, int *sign, void *stream);
+
+/* rns_div (fast = 0, src/rnsdiv.cuh:19-61) or rns_div_fast (fast = 1, :67-114): q, r with the
+ * layout of x (x, d, q, r share mod_stride / num_stride). */
+grns_status grns_div(grns_ctx *ctx, const int *x, const int *d, long long mod_stride, long long num_stride,
+                     size_t count, int *q, int *r, int fast, void *stream);
+
+/* rns_scale2pow (src/rnsscal.cuh:93-119): y_j = x_j scaled by 2^D[j]. */
+grns_status grns_scale2pow(grns_ctx *ctx, const int *x, long long mod_stride, long long num_stride,
+                           const unsigned *D, size_t count, int *y, void *stream);
+
+/* Arg-max of `count` numbers: cuda::rns_max (src/dpp/rnsmax.cuh:129-138).  out_residues[n]
+ * receives the residues of the maximum, out_idx its index (the lowest index among equal
+ * maxima), both in device memory.  workspace: device memory of grns_max_workspace_size bytes. */
+size_t grns_max_workspace_size(const grns_ctx *ctx, size_t count);
+grns_status grns_max(grns_ctx *ctx, const int *x, long long mod_stride, long long num_stride, size_t count,
+                     int *out_residues, long long *out_idx, void *workspace, size_t workspace_bytes, void *stream);
+
+/* mp_int_t (src/types.cuh:45-49) as arrays: digits modulus-major (digits[i * ld + j]),
+ * sign[count], eval[2 * count] (eval[2j] lower, eval[2j+1] upper bound). */
+typedef struct {
+    int *digits;
+    int *sign;
+    grns_er *eval;
+    long long ld;
+} grns_mpint_array;
+
+/* mpint_add / mpint_sub / mpint_mul (src/mpint.cuh:160-278), element-wise z = x op y.
+ * overflow (optional, device, count bytes): mpint_check_overflow (src/mpint.cuh:142-152) as
+ * 0 none, 1 overflow, 2 possible overflow. */
+grns_status grns_mpint_add(grns_ctx *ctx, const grns_mpint_array *x, const grns_mpint_array *y,
+                           const grns_mpint_array *z, size_t count, unsigned char *overflow, void *stream);
+grns_status grns_mpint_sub(grns_ctx *ctx, const grns_mpint_array *x, const grns_mpint_array *y,
+                           const grns_mpint_array *z, size_t count, unsigned char *overflow, void *stream);
+grns_status grns_mpint_mul(grns_ctx *ctx, const grns_mpint_array *x, const grns_mpint_array *y,
+                           const grns_mpint_array *z, size_t count, unsigned char *overflow, void *stream);
+
+/* Version string of the library. */
+const char *grns_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRNS_H */

@@ -1,0 +1,26 @@
+/*
+ * grns_debug.h -- host-side test hooks of libgrns_hip.so.  These run the same __host__ __device__
+ * routines the kernels use, on the CPU, so that they can be compared with a CPU restatement of
+ * the reference without a GPU.  They have no counterpart in the reference.  A context made with
+ * device = -1 (host-only) is enough for all of them.
+ */
+#ifndef GRNS_DEBUG_H
+#define GRNS_DEBUG_H
+#include "grns.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+int grns_debug_eval_hot(const grns_ctx *ctx, const int *x, grns_interval *out, int fast);
+double grns_debug_filter(const grns_ctx *ctx, const int *x, int *regular, int *canon);
+int grns_debug_ceil_log2(double s);
+double grns_debug_mulmod_recip_d(double x, double y, double m, double d);
+int grns_debug_mulmod_recip(int x, int y, int m, double d);
+int grns_debug_addmod_recip(int x, int y, int m, double d);
+double grns_debug_bn_err(double c);
+double grns_debug_fast_scalbn(double x, int n);
+double grns_debug_filter_tau(const grns_ctx *ctx);
+double grns_debug_filter_err(const grns_ctx *ctx);
+#ifdef __cplusplus
+}
+#endif
+#endif

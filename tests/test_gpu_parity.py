@@ -1,0 +1,173 @@
+"""GPU parity: every HIP entry point against the CPU restatement (oracle) on the same seeded
+inputs, bit for bit (interval bounds compared as raw bytes), through the C ABI."""
+import random
+
+import numpy as np
+import pytest
+
+from tests.helpers import SETS, cmp_pairs, div_inputs, eval_inputs, to_res
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return "cuda"
+
+
+@pytest.fixture(scope="module")
+def ctxs(dev):
+    import grns_amd as G
+    from oracle.oracle import Oracle
+    out = {}
+    for n, mod in SETS.items():
+        out[n] = (G.Context(mod, 0), Oracle(mod), mod)
+    yield out
+    for c, o, _ in out.values():
+        c.close()
+        o.close()
+
+
+def _dev(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def _intervals_bytes(t):
+    return t.cpu().numpy().tobytes()
+
+
+@pytest.mark.parametrize("n", [4, 8, 32, 64])
+@pytest.mark.parametrize("fast", [False, True])
+def test_eval_parity(ctxs, dev, n, fast):
+    c, o, mod = ctxs[n]
+    vals = eval_inputs(mod, 3000, seed=10 + n)
+    x = to_res(vals, mod)
+    got = c.eval(_dev(x, dev), fast=fast)
+    ref = o.eval_array(x, fast=fast)
+    g = got.cpu().numpy().view(np.uint8).reshape(len(vals), 32)
+    r = ref.view(np.uint8).reshape(len(vals), 32)
+    bad = np.nonzero((g != r).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first value {vals[bad[0]]}"
+    assert c.status() == 0
+
+
+@pytest.mark.parametrize("n", [8, 32])
+def test_eval_number_major_layout(ctxs, dev, n):
+    c, o, mod = ctxs[n]
+    vals = eval_inputs(mod, 1000, seed=77)
+    x = to_res(vals, mod)
+    xt = _dev(x.T.copy(), dev).t()  # number-major storage viewed as (n, count)
+    got = c.eval(xt)
+    ref = o.eval_array(x)
+    assert got.cpu().numpy().tobytes() == ref.tobytes()
+
+
+@pytest.mark.parametrize("n", [4, 8, 32, 64])
+def test_cmp_parity(ctxs, dev, n):
+    c, o, mod = ctxs[n]
+    xs, ys = cmp_pairs(mod, 4000, seed=20 + n)
+    x, y = to_res(xs, mod), to_res(ys, mod)
+    got = c.cmp(_dev(x, dev), _dev(y, dev)).cpu().numpy()
+    ref = o.cmp_array(x, y)
+    assert (got == ref).all(), f"{(got != ref).sum()} mismatches"
+    truth = np.array([(a > b) - (a < b) for a, b in zip(xs, ys)], np.int32)
+    # the CPU path is exact away from its reciprocal-reduction corner cases
+    assert (ref == truth).mean() > 0.999
+
+
+@pytest.mark.parametrize("n", [8, 32, 64])
+@pytest.mark.parametrize("fast", [True, False])
+def test_div_parity(ctxs, dev, n, fast):
+    c, o, mod = ctxs[n]
+    cnt = {8: 400, 32: 96, 64: 48}[n] if fast else {8: 200, 32: 48, 64: 24}[n]
+    xs, ds = div_inputs(mod, cnt, seed=30 + n)
+    x, d = to_res(xs, mod), to_res(ds, mod)
+    q, r = c.div(_dev(x, dev), _dev(d, dev), fast=fast)
+    rq, rr = o.div_array(x, d, fast=fast)
+    assert (q.cpu().numpy() == rq).all() and (r.cpu().numpy() == rr).all()
+    assert c.status() == 0
+
+
+@pytest.mark.parametrize("n", [8, 32, 64])
+def test_scale2pow_parity(ctxs, dev, n):
+    c, o, mod = ctxs[n]
+    rnd = random.Random(40 + n)
+    vals = eval_inputs(mod, 600, seed=41 + n)
+    L = o.log2M
+    D = np.array([rnd.choice([0, 1, 29, 30, 31, 60, rnd.randrange(1, L + 40)]) for _ in vals], np.uint32)
+    x = to_res(vals, mod)
+    y = c.scale2pow(_dev(x, dev), _dev(D.view(np.int32), dev))
+    ref = o.scale2pow_array(x, D)
+    assert (y.cpu().numpy() == ref).all()
+
+
+@pytest.mark.parametrize("n", [4, 8, 32, 64])
+@pytest.mark.parametrize("case", ["random", "planted", "equal", "small"])
+def test_max_parity(ctxs, dev, n, case):
+    c, o, mod = ctxs[n]
+    from grns_amd.moduli import product
+    M = product(mod)
+    rnd = random.Random(50 + n)
+    if case == "small":
+        vals = [rnd.randrange(M) for _ in range(3)]
+    elif case == "equal":
+        v = rnd.randrange(M)
+        vals = [v] * 1500
+    else:
+        vals = [rnd.randrange(M // 2) for _ in range(20000)]
+        if case == "planted":
+            V = M // 2 - 1
+            idx = rnd.sample(range(len(vals)), 40)
+            vals[idx[0]] = V
+            for k in idx[1:]:
+                vals[k] = V - 1
+    x = to_res(vals, mod)
+    res, idx = c.max(_dev(x, dev))
+    ridx, rres = o.max_array(x)
+    truth = max(range(len(vals)), key=lambda k: (vals[k], -k))
+    assert int(idx.item()) == truth == ridx
+    assert (res.cpu().numpy() == rres).all()
+
+
+def _mpint_operands(mod, o, count, seed, bits):
+    rnd = random.Random(seed)
+    mags = [rnd.randrange(1 << rnd.randrange(1, bits)) for _ in range(count)]
+    mags[:4] = [0, 1, 55, 13]
+    signs = [rnd.randrange(2) for _ in range(count)]
+    signs[:4] = [0, 0, 1, 0]
+    d = to_res(mags, mod)
+    ev = o.eval_array(d)
+    e = np.zeros((count, 2), dtype=ev["low"].dtype)
+    e[:, 0] = ev["low"]
+    e[:, 1] = ev["upp"]
+    for j in range(count):  # mpint_set_mpz: sign 0 for zero
+        if mags[j] == 0:
+            signs[j] = 0
+    return d, np.array(signs, np.int32), e
+
+
+@pytest.mark.parametrize("n", [8, 32, 64])
+@pytest.mark.parametrize("op", ["add", "sub", "mul"])
+def test_mpint_parity(ctxs, dev, n, op):
+    c, o, mod = ctxs[n]
+    half = (o.log2M - 2) // 2
+    cnt = 3000
+    xd, xs, xe = _mpint_operands(mod, o, cnt, 60 + n, half)
+    yd, ys, ye = _mpint_operands(mod, o, cnt, 70 + n, half)
+    rd, rs, re, ovf = o.mpint_array(op, xd, xs, xe, yd, ys, ye)
+
+    def dev_triple(d, s, e):
+        return (_dev(d, dev), _dev(s, dev), _dev(e.view(np.float64).reshape(cnt, 2, 2), dev))
+
+    X, Y = dev_triple(xd, xs, xe), dev_triple(yd, ys, ye)
+    Z = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
+    of = torch.empty(cnt, dtype=torch.uint8, device=dev)
+    c.mpint(op, X, Y, Z, cnt, overflow=of)
+    assert (Z[0].cpu().numpy() == rd).all()
+    assert (Z[1].cpu().numpy() == rs).all()
+    assert Z[2].cpu().numpy().tobytes() == re.tobytes()
+    assert (of.cpu().numpy() == ovf).all()
