@@ -822,6 +822,59 @@ long orc_max_array(const orc_ctx *c, const int *x, size_t ld, size_t count, int 
     free(best_mr);
     return bi;
 }
+/* The reference's rns_max pipeline run on the CPU path (src/dpp/rnsmax.cuh:24-101 with
+ * rns_eval_compute of src/rnseval.cuh:29-102): evaluate every element, then reduce with
+ * rns_max_cmp (interval test, residue equality, MRC); strict ">" keeps the incumbent.
+ * Used as the timed CPU baseline of the arg-max workload. */
+long orc_max_eval_array(const orc_ctx *c, const int *x, size_t ld, size_t count, int nthreads) {
+    int n = c->n;
+    if (count == 0) return -1;
+    int nt = orc_threads(nthreads);
+    long *best = (long *)malloc(sizeof(long) * nt);
+    orc_interval_t *bev = (orc_interval_t *)malloc(sizeof(orc_interval_t) * nt);
+    long long cnt = (long long)count;
+#pragma omp parallel num_threads(nt)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        int v[ORC_MAX_MODULI], w[ORC_MAX_MODULI];
+        orc_interval_t ev;
+        best[tid] = -1;
+#pragma omp for schedule(static)
+        for (long long j = 0; j < cnt; j++) {
+            orc_gather(c, v, x, ld, (size_t)j);
+            orc_eval(c, &ev, v);
+            int r = 1;
+            if (best[tid] >= 0) {
+                if (orc_er_ucmp(&ev.low, &bev[tid].upp) > 0) r = 1;
+                else if (orc_er_ucmp(&bev[tid].low, &ev.upp) > 0) r = -1;
+                else {
+                    orc_gather(c, w, x, ld, (size_t)best[tid]);
+                    r = memcmp(v, w, sizeof(int) * n) == 0 ? 0 : orc_mrc_compare(c, v, w);
+                }
+            }
+            if (r == 1) { best[tid] = (long)j; bev[tid] = ev; }
+        }
+    }
+    long bi = -1;
+    int bt = -1;
+    for (int t = 0; t < nt; t++) {
+        if (best[t] < 0) continue;
+        if (bi < 0) { bi = best[t]; bt = t; continue; }
+        int v[ORC_MAX_MODULI], w[ORC_MAX_MODULI];
+        orc_gather(c, v, x, ld, (size_t)best[t]);
+        orc_gather(c, w, x, ld, (size_t)bi);
+        int r = orc_mrc_compare(c, v, w);
+        if (r > 0) { bi = best[t]; bt = t; }
+    }
+    (void)bt;
+    free(best);
+    free(bev);
+    return bi;
+}
+
 void orc_mpint_array(const orc_ctx *c, int op, size_t ld, size_t count,
                      const int *xd, const int *xs, const orc_er_t *xe,
                      const int *yd, const int *ys, const orc_er_t *ye,

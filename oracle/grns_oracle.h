@@ -120,6 +120,8 @@ void orc_div_array(const orc_ctx *c, const int *x, const int *d, size_t ld, size
 void orc_scale2pow_array(const orc_ctx *c, const int *x, const unsigned int *D, size_t ld, size_t count, int *y, int nthreads);
 /* Arg-max with rns_max_cmp semantics; ties resolve to the lowest index. Returns the index. */
 long orc_max_array(const orc_ctx *c, const int *x, size_t ld, size_t count, int *out_residues, int nthreads);
+/* The reference's rns_max pipeline (evaluate, then reduce with rns_max_cmp) on the CPU path. */
+long orc_max_eval_array(const orc_ctx *c, const int *x, size_t ld, size_t count, int nthreads);
 /* mpint elementwise op over SoA arrays: op 0 add, 1 sub, 2 mul. Digits modulus-major with ld. */
 void orc_mpint_array(const orc_ctx *c, int op, size_t ld, size_t count,
                      const int *xd, const int *xs, const orc_er_t *xe,

@@ -126,6 +126,8 @@ def lib():
         L.orc_scale2pow_array.argtypes = [CP, P, P, C.c_size_t, C.c_size_t, P, C.c_int]
         L.orc_max_array.restype = C.c_long
         L.orc_max_array.argtypes = [CP, P, C.c_size_t, C.c_size_t, P, C.c_int]
+        L.orc_max_eval_array.restype = C.c_long
+        L.orc_max_eval_array.argtypes = [CP, P, C.c_size_t, C.c_size_t, C.c_int]
         L.orc_mpint_array.argtypes = [CP, C.c_int, C.c_size_t, C.c_size_t, P, P, P, P, P, P, P, P, P, P, C.c_int]
         _lib = L
     return _lib
@@ -255,6 +257,12 @@ class Oracle:
         res = np.zeros(self.n, np.int32)
         idx = lib().orc_max_array(self._p, _ptr(x), cnt, cnt, _ptr(res), nthreads)
         return idx, res
+
+    def max_eval_array(self, x, nthreads=0):
+        """Arg-max by the reference's evaluate-then-reduce pipeline (CPU path)."""
+        x = _i32(x)
+        cnt = x.shape[1]
+        return lib().orc_max_eval_array(self._p, _ptr(x), cnt, cnt, nthreads)
 
     def mpint_array(self, op, xd, xs, xe, yd, ys, ye, nthreads=0):
         """op: 'add' | 'sub' | 'mul'.  xe/ye: ER_DTYPE arrays of shape (count, 2)."""

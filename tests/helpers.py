@@ -45,8 +45,8 @@ def eval_inputs(moduli, count, seed):
         return list(range(M))
     vals = edge_values(M)
     vals += [1 << j for j in range(0, M.bit_length() - 1, 7)]
-    vals += [rnd.randrange(1 << rnd.randrange(1, 64)) for _ in range(count // 10)]
-    vals += [M - 1 - rnd.randrange(1 << 40) for _ in range(count // 20)]
+    vals += [rnd.randrange(1 << rnd.randrange(1, 64)) % M for _ in range(count // 10)]
+    vals += [(M - 1 - rnd.randrange(1 << 40)) % M for _ in range(count // 20)]
     while len(vals) < count:
         vals.append(rnd.randrange(M))
     return vals[:count]

@@ -1,0 +1,167 @@
+"""CPU checks of libgrns_hip.so (no GPU needed):
+
+* the library loads and exports every function the public headers declare;
+* host-only contexts (device = -1) hold the same constants as the CPU restatement;
+* the hot-path routines, which are __host__ __device__, run on the CPU through the debug hooks and
+  agree bit for bit with the CPU restatement;
+* the fused bn_err (one fma) returns the reference's unfused RN(RN(PHI1*|c|) + ETA) across all
+  exponent ranges the kernels can see, and the glibc-equivalent ceil(log2(.)) agrees with libm.
+"""
+import math
+import os
+import random
+import re
+import struct
+
+import numpy as np
+import pytest
+
+import grns_amd as G
+from oracle.oracle import Oracle, lib as olib
+from tests.helpers import SETS, eval_inputs, to_res
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PHI1 = 2.0 ** -53 * (1 + 2 * 2.0 ** -53)
+ETA = 2.0 ** -1074
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(grns_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_exports_every_declared_symbol():
+    L = G.lib()
+    names = _declared("grns.h") + _declared("grns_debug.h")
+    assert len(names) >= 20
+    missing = [nm for nm in names if not hasattr(L, nm)]
+    assert not missing, missing
+    assert G.version().startswith("grns_amd")
+
+
+def test_bad_arguments_rejected():
+    L = G.lib()
+    import ctypes as C
+    h = C.c_void_p()
+    arr = np.array([3, 5], np.int32)
+    assert L.grns_ctx_create(arr.ctypes.data, 0, -1, C.byref(h)) == 1
+    arr = np.array([3, 1], np.int32)
+    assert L.grns_ctx_create(arr.ctypes.data, 2, -1, C.byref(h)) == 1
+    c = G.Context(SETS[8], device=-1)
+    # device entry points refuse a host-only context instead of touching the GPU
+    assert L.grns_eval(c.handle, None, 1, 1, 0, None, 0, None) in (0, 1)
+    assert L.grns_eval(c.handle, C.c_void_p(8), 1, 1, 1, C.c_void_p(8), 0, None) == 1
+
+
+@pytest.mark.parametrize("n", [4, 8, 32, 64])
+def test_host_context_constants_match_oracle(n):
+    c = G.Context(SETS[n], device=-1)
+    o = Oracle(SETS[n])
+    oc, cc = o.constants(), c.constants()
+    for k in cc:
+        assert oc[k] == cc[k], k
+    p2, mi = c.tables()
+    assert (p2 == o.pow2_table()).all() and (mi == o.mrc_inv()).all()
+
+
+@pytest.mark.parametrize("n", [4, 8, 32, 64])
+def test_host_hot_path_matches_oracle(n):
+    mod = SETS[n]
+    c = G.Context(mod, device=-1)
+    o = Oracle(mod)
+    vals = eval_inputs(mod, 2500, seed=100 + n)
+    hot = 0
+    for fast in (False, True):
+        ref = o.eval_array(to_res(vals, mod), fast=fast)
+        for k, v in enumerate(vals):
+            ok, iv = c.debug_eval_hot([v % m for m in mod], fast)
+            if ok == 1:
+                hot += 1
+                assert iv.tobytes() == ref[k].tobytes(), v
+    assert hot > len(vals)
+
+
+@pytest.mark.parametrize("n", [8, 32, 64])
+def test_filter_bounds(n):
+    """The comparison/arg-max filter: |F - x/M mod 1| within the stated bound, and `regular`
+    only when the CPU-path evaluation takes its common path with canonical reductions."""
+    from fractions import Fraction
+    mod = SETS[n]
+    M = G.product(mod)
+    c = G.Context(mod, device=-1)
+    o = Oracle(mod)
+    err = G.lib().grns_debug_filter_err(c.handle)
+    vals = eval_inputs(mod, 1500, seed=200 + n)
+    for v in vals:
+        r = [v % m for m in mod]
+        Fv, reg, can = c.debug_filter(r)
+        ex = Fraction(v, M)
+        d = abs(Fraction(Fv) - ex)
+        d = min(d, abs(d - 1))
+        assert d <= Fraction(err), v
+        if reg:
+            ok, _ = c.debug_eval_hot(r, False)
+            assert ok == 1
+            iv = o.eval1(r)
+            lo = Fraction(float(iv["low"]["frac"])) * Fraction(2) ** int(iv["low"]["exp"])
+            up = Fraction(float(iv["upp"]["frac"])) * Fraction(2) ** int(iv["upp"]["exp"])
+            assert lo <= ex <= up
+
+
+def _rn_unfused(c):
+    return PHI1 * abs(c) + ETA  # Python floats: IEEE binary64, round to nearest, no contraction
+
+
+def test_fused_bound_equals_reference():
+    L = G.lib()
+    rnd = random.Random(7)
+    cases = [0.0, -0.0, ETA, 2 * ETA, 3 * ETA, 1.5, 3.0, 2.0 ** -52, 1.0 - 2.0 ** -53]
+    cases += [1.5 * 2.0 ** k for k in range(-900, 1000, 7)]
+    cases += [k * ETA for k in range(1, 2000, 13)]
+    for _ in range(40000):
+        e = rnd.randrange(-917, 1020)
+        cases.append(rnd.choice([-1, 1]) * rnd.uniform(1, 2) * 2.0 ** e)
+    for _ in range(4000):  # significands right at rounding midpoints of PHI1*c
+        m = rnd.randrange(1 << 52, 1 << 53)
+        cases.append(m * 2.0 ** rnd.randrange(-960, 900))
+    for c in cases:
+        assert L.grns_debug_bn_err(c) == _rn_unfused(c), c
+
+
+def test_ceil_log2_matches_libm():
+    L = G.lib()
+    rnd = random.Random(8)
+    cases = []
+    for e in range(-1074, 60):
+        p = 2.0 ** e
+        cases.append(p)
+        for k in range(1, 40):
+            cases.append(p * (1 + k * 2.0 ** -52) if e > -1022 else p + k * ETA)
+        cases.append(p * (1 - 2.0 ** -53))
+        cases.append(p * rnd.uniform(1, 2))
+    for s in cases:
+        if s <= 0 or math.isinf(s):
+            continue
+        assert L.grns_debug_ceil_log2(s) == math.ceil(math.log2(s)), s
+
+
+def test_scalbn_and_reductions_match_oracle():
+    L = G.lib()
+    O = olib()
+    rnd = random.Random(9)
+    for _ in range(20000):
+        x = rnd.choice([0.0, 1.0, -1.5, rnd.uniform(1, 2), -rnd.uniform(1, 2)])
+        n = rnd.choice([rnd.randrange(-2100, 1100), -1024, -1023, -1025, -2047, -2048, 1023])
+        a, b = L.grns_debug_fast_scalbn(x, n), O.orc_fast_scalbn(x, n)
+        assert struct.pack("<d", a) == struct.pack("<d", b), (x, n)
+    mods = SETS[32][:6] + [3, 5, 7, 11, 2147483647]
+    for m in mods:
+        d = 1.0 / m
+        for _ in range(4000):
+            x = rnd.randrange(-m, m + 2) if rnd.random() < 0.3 else rnd.randrange(m)
+            y = rnd.randrange(m)
+            want = O.orc_mod_mul(x, y, m, d)
+            assert L.grns_debug_mulmod_recip(x, y, m, d) == want
+            assert L.grns_debug_mulmod_recip_d(float(x), float(y), float(m), d) == float(want)
+            assert L.grns_debug_addmod_recip(x, y, m, d) == O.orc_mod_add(x, y, m, d)
