@@ -46,12 +46,30 @@ class MpintArray(C.Structure):
     _fields_ = [("digits", C.c_void_p), ("sign", C.c_void_p), ("eval", C.c_void_p), ("ld", C.c_longlong)]
 
 
+class BatchTask(C.Structure):
+    _fields_ = [("op", C.c_int), ("count", C.c_size_t),
+                ("a", C.c_void_p), ("a_mod_stride", C.c_longlong), ("a_num_stride", C.c_longlong),
+                ("b", C.c_void_p), ("b_mod_stride", C.c_longlong), ("b_num_stride", C.c_longlong),
+                ("out0", C.c_void_p), ("out1", C.c_void_p), ("D", C.c_void_p),
+                ("mx", MpintArray), ("my", MpintArray), ("mz", MpintArray), ("overflow", C.c_void_p)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("tile", C.c_int), ("blocks_per_cu", C.c_int), ("threads_per_number", C.c_int),
+                ("use_graph", C.c_int)]
+
+
+OPS = {"eval": 0, "eval_fast": 1, "cmp": 2, "mpint_add": 3, "mpint_sub": 4, "mpint_mul": 5, "div": 6,
+       "div_fast": 7, "scale2pow": 8}
+
 _lib = None
 
 EXPORTS = [
     "grns_version", "grns_ctx_create", "grns_ctx_destroy", "grns_ctx_n", "grns_ctx_log2m", "grns_ctx_constants",
     "grns_ctx_tables", "grns_ctx_status", "grns_eval", "grns_cmp", "grns_div", "grns_scale2pow",
     "grns_max_workspace_size", "grns_max", "grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul",
+    "grns_batch_create", "grns_batch_set_config", "grns_batch_get_config", "grns_batch_tune", "grns_batch_launch",
+    "grns_batch_destroy",
 ]
 DEBUG_EXPORTS = [
     "grns_debug_eval_hot", "grns_debug_filter", "grns_debug_ceil_log2", "grns_debug_mulmod_recip_d",
@@ -87,6 +105,12 @@ def lib():
     L.grns_max.argtypes = [P, P, LL, LL, SZ, P, P, P, SZ, P]
     for nm in ("grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul"):
         getattr(L, nm).argtypes = [P, C.POINTER(MpintArray), C.POINTER(MpintArray), C.POINTER(MpintArray), SZ, P, P]
+    L.grns_batch_create.argtypes = [P, C.POINTER(BatchTask), I, C.POINTER(P)]
+    L.grns_batch_set_config.argtypes = [P, C.POINTER(BatchConfig)]
+    L.grns_batch_get_config.argtypes = [P, C.POINTER(BatchConfig)]
+    L.grns_batch_tune.argtypes = [P, P, I, C.POINTER(BatchConfig), C.POINTER(C.c_double)]
+    L.grns_batch_launch.argtypes = [P, P]
+    L.grns_batch_destroy.argtypes = [P]
     L.grns_debug_eval_hot.argtypes = [P, P, P, I]
     L.grns_debug_filter.restype = D
     L.grns_debug_filter.argtypes = [P, P, P, P]
@@ -281,6 +305,71 @@ class Context:
         rg, cn = C.c_int(), C.c_int()
         F = lib().grns_debug_filter(self._h, _np_ptr(a), C.byref(rg), C.byref(cn))
         return F, bool(rg.value), bool(cn.value)
+
+
+def make_task(op, count, a=None, b=None, out0=None, out1=None, D=None, mx=None, my=None, mz=None, overflow=None):
+    """grns_batch_task from device tensors.  a/b/out0/out1: (n, count)-shaped residue views
+    (or the interval / sign outputs); mx/my/mz: (digits, sign, eval) triples for mpint ops."""
+    t = BatchTask()
+    t.op = OPS[op] if isinstance(op, str) else int(op)
+    t.count = int(count)
+    if a is not None:
+        t.a, t.a_mod_stride, t.a_num_stride = a.data_ptr(), a.stride(0), a.stride(1)
+    if b is not None:
+        t.b, t.b_mod_stride, t.b_num_stride = b.data_ptr(), b.stride(0), b.stride(1)
+    t.out0 = out0.data_ptr() if out0 is not None else None
+    t.out1 = out1.data_ptr() if out1 is not None else None
+    t.D = D.data_ptr() if D is not None else None
+    if mx is not None:
+        t.mx, t.my, t.mz = mpint_array(*mx), mpint_array(*my), mpint_array(*mz)
+    t.overflow = overflow.data_ptr() if overflow is not None else None
+    return t
+
+
+class Batch:
+    """A fixed set of tasks on fixed buffers (grns_batch_*): one persistent launch per call."""
+
+    def __init__(self, ctx: Context, tasks):
+        arr = (BatchTask * max(1, len(tasks)))(*tasks)
+        h = C.c_void_p()
+        _check(lib().grns_batch_create(ctx.handle, arr, len(tasks), C.byref(h)), "grns_batch_create")
+        self._h = h
+        self.ctx = ctx
+        self.tasks = tasks
+
+    def config(self):
+        c = BatchConfig()
+        _check(lib().grns_batch_get_config(self._h, C.byref(c)), "grns_batch_get_config")
+        return {"tile": c.tile, "blocks_per_cu": c.blocks_per_cu, "threads_per_number": c.threads_per_number,
+                "use_graph": c.use_graph}
+
+    def set_config(self, tile=None, blocks_per_cu=None, threads_per_number=None, use_graph=None):
+        cur = self.config()
+        c = BatchConfig(tile if tile is not None else cur["tile"],
+                        blocks_per_cu if blocks_per_cu is not None else cur["blocks_per_cu"],
+                        threads_per_number if threads_per_number is not None else cur["threads_per_number"],
+                        int(use_graph) if use_graph is not None else cur["use_graph"])
+        _check(lib().grns_batch_set_config(self._h, C.byref(c)), "grns_batch_set_config")
+
+    def tune(self, reps=3, stream=None):
+        c = BatchConfig()
+        ms = C.c_double()
+        _check(lib().grns_batch_tune(self._h, _stream_ptr(stream), reps, C.byref(c), C.byref(ms)), "grns_batch_tune")
+        return self.config(), ms.value
+
+    def launch(self, stream=None):
+        _check(lib().grns_batch_launch(self._h, _stream_ptr(stream)), "grns_batch_launch")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().grns_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def version() -> str:

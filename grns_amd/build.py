@@ -16,8 +16,8 @@ LIB = os.path.join(HERE, "libgrns_hip.so")
 WORKLOAD_LIB = os.path.join(HERE, "libgrns_workload.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["grns_lib.hip"]
-HEADERS = ["grns_device.hpp", "grns_kernels.hpp"]
+SOURCES = ["grns_lib.hip", "grns_batch.hip"]
+HEADERS = ["grns_device.hpp", "grns_kernels.hpp", "grns_internal.hpp"]
 
 FLAGS = [
     "--offload-arch=gfx950",

@@ -1,0 +1,420 @@
+// grns_batch.hip -- batched form: many arrays of uneven length (and several operations of one
+// moduli set) in one persistent launch.  Work is cut into tiles of `tile` numbers; persistent
+// blocks take tiles from an atomic counter (work stealing across tasks, so short and long arrays
+// mix freely), look up the task of the tile by binary search over the tiles' prefix, and run the
+// same per-number code as the per-call kernels, so every result is identical to theirs.  The
+// counter is reset by a one-thread kernel in the same stream, so the whole sequence can be
+// captured in a hipGraph and replayed.  An autotuner times candidate tile sizes, persistent
+// blocks per CU and the lane mapping (one lane or one group of lanes per number) on the batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "grns_internal.hpp"
+
+using namespace grns;
+
+namespace grns {
+
+struct BatchTask {
+    int op;
+    int pad;
+    long long count;
+    long long tile0;  // first tile index of this task within its list
+    const int *a;
+    long long asm_, asn;
+    const int *b;
+    long long bsm, bsn;
+    void *o0;
+    void *o1;
+    const unsigned *D;
+    MpintArgs mp;
+};
+
+GRNS_D int find_task(const BatchTask *__restrict__ tasks, int ntasks, long long t) {
+    int lo = 0, hi = ntasks - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (tasks[mid].tile0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Fetch the next tile for the block: returns false when the work is exhausted.
+GRNS_D bool next_tile(const BatchTask *__restrict__ tasks, int ntasks, long long total, unsigned long long *counter,
+                      long long &t, int &k) {
+    __shared__ long long s_t;
+    __shared__ int s_k;
+    if (threadIdx.x == 0) {
+        long long v = (long long)atomicAdd(counter, 1ull);
+        s_t = v;
+        s_k = v < total ? find_task(tasks, ntasks, v) : 0;
+    }
+    __syncthreads();
+    t = s_t;
+    k = s_k;
+    __syncthreads();
+    return t < total;
+}
+
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_batch_thread(const DevCtx *__restrict__ c, const BatchTask *__restrict__ tasks,
+                                                        int ntasks, long long total, int tile,
+                                                        unsigned long long *counter, int *__restrict__ status) {
+    long long t;
+    int k;
+    while (next_tile(tasks, ntasks, total, counter, t, k)) {
+        const BatchTask &T = tasks[k];
+        long long e0 = (t - T.tile0) * tile;
+        long long e1 = e0 + tile < T.count ? e0 + tile : T.count;
+        switch (T.op) {
+            case GRNS_OP_EVAL:
+            case GRNS_OP_EVAL_FAST: {
+                interval_t *o = reinterpret_cast<interval_t *>(T.o0);
+                for (long long base = e0; base < e1; base += BLOCK) {
+                    if (T.op == GRNS_OP_EVAL) eval_step<N, false>(c, T.a, T.asm_, T.asn, base + threadIdx.x, e1, o, status);
+                    else eval_step<N, true>(c, T.a, T.asm_, T.asn, base + threadIdx.x, e1, o, status);
+                }
+                break;
+            }
+            case GRNS_OP_CMP:
+                for (long long base = e0; base < e1; base += BLOCK)
+                    cmp_step<N>(c, T.a, T.asm_, T.asn, T.b, T.bsm, T.bsn, base + threadIdx.x, e1,
+                                reinterpret_cast<int *>(T.o0), status);
+                break;
+            case GRNS_OP_MPINT_ADD:
+                for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 0>(c, T.mp, j);
+                break;
+            case GRNS_OP_MPINT_SUB:
+                for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 1>(c, T.mp, j);
+                break;
+            case GRNS_OP_MPINT_MUL:
+                for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 2>(c, T.mp, j);
+                break;
+            default:
+                break;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_batch_group(const DevCtx *__restrict__ c, int G,
+                                                       const BatchTask *__restrict__ tasks, int ntasks, long long total,
+                                                       int tile, unsigned long long *counter, int *__restrict__ status) {
+    Group g = make_group(G, c->n);
+    const int P = 64 / G;
+    const long long wave = threadIdx.x >> 6;
+    const long long gid = (threadIdx.x & 63) / G;
+    long long t;
+    int k;
+    while (next_tile(tasks, ntasks, total, counter, t, k)) {
+        const BatchTask &T = tasks[k];
+        long long e0 = (t - T.tile0) * tile;
+        long long e1 = e0 + tile < T.count ? e0 + tile : T.count;
+        for (long long j0 = e0 + wave * P; j0 < e1; j0 += (BLOCK / 64) * P) {
+            long long j = j0 + gid;
+            if (j >= e1) continue;
+            int st = 0;
+            const bool act = g.gl < g.L;
+            long long oa = j * T.asn + (long long)g.gl * T.asm_;
+            switch (T.op) {
+                case GRNS_OP_EVAL:
+                case GRNS_OP_EVAL_FAST: {
+                    int xi = act ? T.a[oa] : 0;
+                    interval_t r = (T.op == GRNS_OP_EVAL) ? group_eval<false>(c, g, xi, st) : group_eval<true>(c, g, xi, st);
+                    if (g.gl == 0) store_interval(reinterpret_cast<interval_t *>(T.o0) + j, r);
+                    break;
+                }
+                case GRNS_OP_CMP: {
+                    int xi = act ? T.a[oa] : 0;
+                    int yi = act ? T.b[j * T.bsn + (long long)g.gl * T.bsm] : 0;
+                    int r = group_cmp(c, g, xi, yi, st);
+                    if (g.gl == 0) reinterpret_cast<int *>(T.o0)[j] = r;
+                    break;
+                }
+                case GRNS_OP_MPINT_ADD: group_mpint<0>(c, g, T.mp, j); break;
+                case GRNS_OP_MPINT_SUB: group_mpint<1>(c, g, T.mp, j); break;
+                case GRNS_OP_MPINT_MUL: group_mpint<2>(c, g, T.mp, j); break;
+                case GRNS_OP_DIV:
+                case GRNS_OP_DIV_FAST: {
+                    int xi = act ? T.a[oa] : 0;
+                    int di = act ? T.b[oa] : 0;
+                    int q, r;
+                    if (T.op == GRNS_OP_DIV) group_div<false>(c, g, xi, di, q, r, st);
+                    else group_div<true>(c, g, xi, di, q, r, st);
+                    if (act) {
+                        reinterpret_cast<int *>(T.o0)[oa] = q;
+                        reinterpret_cast<int *>(T.o1)[oa] = r;
+                    }
+                    break;
+                }
+                case GRNS_OP_SCALE2POW: {
+                    int xi = act ? T.a[oa] : 0;
+                    int v = group_scale2pow(c, g, xi, T.D[j]);
+                    if (act) reinterpret_cast<int *>(T.o0)[oa] = v;
+                    break;
+                }
+                default:
+                    break;
+            }
+            if (g.gl == 0 && st) atomicOr(status, st);
+        }
+    }
+}
+
+__global__ void k_batch_reset(unsigned long long *counters) {
+    if (threadIdx.x < 2) counters[threadIdx.x] = 0;
+}
+
+}  // namespace grns
+
+struct grns_batch {
+    grns_ctx *ctx = nullptr;
+    std::vector<grns_batch_task> user;
+    std::vector<BatchTask> thr, grp;   // task lists of the two mappings, tile0 filled in
+    long long thr_tiles = 0, grp_tiles = 0;
+    BatchTask *d_tasks = nullptr;
+    unsigned long long *d_counters = nullptr;
+    int cap = 0;
+    grns_batch_config cfg{1024, 4, 1, 0};
+    int cus = 256;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+bool thread_kernel_available(int n) { return n == 4 || n == 8 || n == 16 || n == 32 || n == 64; }
+
+bool elementwise(int op) { return op >= GRNS_OP_EVAL && op <= GRNS_OP_MPINT_MUL; }
+
+BatchTask to_dev(const grns_batch_task &u) {
+    BatchTask t;
+    std::memset(&t, 0, sizeof(t));
+    t.op = u.op;
+    t.count = (long long)u.count;
+    t.a = u.a; t.asm_ = u.a_mod_stride; t.asn = u.a_num_stride;
+    t.b = u.b; t.bsm = u.b_mod_stride; t.bsn = u.b_num_stride;
+    t.o0 = u.out0; t.o1 = u.out1; t.D = u.D;
+    t.mp.xd = u.mx.digits; t.mp.xs = u.mx.sign; t.mp.xe = reinterpret_cast<const er_t *>(u.mx.eval);
+    t.mp.yd = u.my.digits; t.mp.ys = u.my.sign; t.mp.ye = reinterpret_cast<const er_t *>(u.my.eval);
+    t.mp.rd = u.mz.digits; t.mp.rs = u.mz.sign; t.mp.re = reinterpret_cast<er_t *>(u.mz.eval);
+    t.mp.ovf = u.overflow; t.mp.ld = u.mx.ld;
+    return t;
+}
+
+void drop_graph(grns_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+grns_status rebuild(grns_batch *b) {
+    drop_graph(b);
+    b->thr.clear();
+    b->grp.clear();
+    const bool thread_ok = thread_kernel_available(b->ctx->n) && b->cfg.threads_per_number == 1;
+    long long tile = b->cfg.tile;
+    for (const auto &u : b->user) {
+        if (u.count == 0) continue;
+        BatchTask t = to_dev(u);
+        auto &lst = (thread_ok && elementwise(u.op)) ? b->thr : b->grp;
+        lst.push_back(t);
+    }
+    b->thr_tiles = 0;
+    for (auto &t : b->thr) { t.tile0 = b->thr_tiles; b->thr_tiles += (t.count + tile - 1) / tile; }
+    b->grp_tiles = 0;
+    for (auto &t : b->grp) { t.tile0 = b->grp_tiles; b->grp_tiles += (t.count + tile - 1) / tile; }
+    std::vector<BatchTask> all(b->thr);
+    all.insert(all.end(), b->grp.begin(), b->grp.end());
+    if (!all.empty() &&
+        hipMemcpy(b->d_tasks, all.data(), sizeof(BatchTask) * all.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return GRNS_ERR_HIP;
+    return GRNS_OK;
+}
+
+grns_status enqueue(grns_batch *b, hipStream_t s) {
+    grns_ctx *c = b->ctx;
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, b->d_counters);
+    dim3 grid((unsigned)(b->cus * b->cfg.blocks_per_cu)), blk(BLOCK);
+    int nthr = (int)b->thr.size(), ngrp = (int)b->grp.size();
+    if (nthr) {
+        switch (c->n) {
+#define GRNS_BT(NN)                                                                                                  \
+    case NN:                                                                                                         \
+        hipLaunchKernelGGL((k_batch_thread<NN>), grid, blk, 0, s, c->d, b->d_tasks, nthr, b->thr_tiles, b->cfg.tile, \
+                           b->d_counters, c->d_status);                                                              \
+        break;
+            GRNS_BT(4)
+            GRNS_BT(8)
+            GRNS_BT(16)
+            GRNS_BT(32)
+            GRNS_BT(64)
+#undef GRNS_BT
+            default:
+                return GRNS_ERR_UNSUPPORTED;
+        }
+    }
+    if (ngrp) {
+        hipLaunchKernelGGL(k_batch_group, grid, blk, 0, s, c->d, groups_for(c->n), b->d_tasks + nthr, ngrp,
+                           b->grp_tiles, b->cfg.tile, b->d_counters + 1, c->d_status);
+    }
+    return hipGetLastError() == hipSuccess ? GRNS_OK : GRNS_ERR_HIP;
+}
+
+}  // namespace
+
+extern "C" {
+
+grns_status grns_batch_create(grns_ctx *ctx, const grns_batch_task *tasks, int ntasks, grns_batch **out) {
+    if (!ctx || !out || ntasks < 0 || (ntasks && !tasks) || ctx->device < 0 || ctx->n > 64) return GRNS_ERR_ARG;
+    for (int i = 0; i < ntasks; i++) {
+        const grns_batch_task &u = tasks[i];
+        if (u.op < GRNS_OP_EVAL || u.op > GRNS_OP_SCALE2POW) return GRNS_ERR_ARG;
+        if (u.count == 0) continue;
+        bool mp = u.op >= GRNS_OP_MPINT_ADD && u.op <= GRNS_OP_MPINT_MUL;
+        if (mp) {
+            if (!u.mx.digits || !u.my.digits || !u.mz.digits || u.mx.ld != u.my.ld || u.mx.ld != u.mz.ld ||
+                u.mx.ld < (long long)u.count)
+                return GRNS_ERR_ARG;
+        } else {
+            if (!u.a || !u.out0) return GRNS_ERR_ARG;
+            if ((u.op == GRNS_OP_CMP || u.op == GRNS_OP_DIV || u.op == GRNS_OP_DIV_FAST) && !u.b) return GRNS_ERR_ARG;
+            if ((u.op == GRNS_OP_DIV || u.op == GRNS_OP_DIV_FAST) && !u.out1) return GRNS_ERR_ARG;
+            if (u.op == GRNS_OP_SCALE2POW && !u.D) return GRNS_ERR_ARG;
+        }
+    }
+    grns_batch *b = new (std::nothrow) grns_batch();
+    if (!b) return GRNS_ERR_OOM;
+    b->ctx = ctx;
+    b->user.assign(tasks, tasks + ntasks);
+    b->cap = ntasks > 0 ? ntasks : 1;
+    int dev = ctx->device;
+    (void)hipSetDevice(dev);
+    if (hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) b->cus = 256;
+    if (!thread_kernel_available(ctx->n)) b->cfg.threads_per_number = 0;
+    if (hipMalloc(&b->d_tasks, sizeof(BatchTask) * b->cap) != hipSuccess ||
+        hipMalloc(&b->d_counters, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        grns_batch_destroy(b);
+        return GRNS_ERR_OOM;
+    }
+    grns_status st = rebuild(b);
+    if (st != GRNS_OK) {
+        grns_batch_destroy(b);
+        return st;
+    }
+    *out = b;
+    return GRNS_OK;
+}
+
+grns_status grns_batch_set_config(grns_batch *b, const grns_batch_config *cfg) {
+    if (!b || !cfg || cfg->tile < 1 || cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 64) return GRNS_ERR_ARG;
+    grns_batch_config nc = *cfg;
+    if (!thread_kernel_available(b->ctx->n)) nc.threads_per_number = 0;
+    if (nc.threads_per_number != 0) nc.threads_per_number = 1;
+    bool reshape = nc.tile != b->cfg.tile || nc.threads_per_number != b->cfg.threads_per_number;
+    bool regraph = reshape || nc.blocks_per_cu != b->cfg.blocks_per_cu || nc.use_graph != b->cfg.use_graph;
+    b->cfg = nc;
+    if (reshape) return rebuild(b);
+    if (regraph) drop_graph(b);
+    return GRNS_OK;
+}
+
+grns_status grns_batch_get_config(const grns_batch *b, grns_batch_config *cfg) {
+    if (!b || !cfg) return GRNS_ERR_ARG;
+    *cfg = b->cfg;
+    return GRNS_OK;
+}
+
+grns_status grns_batch_launch(grns_batch *b, void *stream) {
+    if (!b) return GRNS_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    (void)hipSetDevice(b->ctx->device);
+    if (b->thr.empty() && b->grp.empty()) return GRNS_OK;
+    if (!b->cfg.use_graph) return enqueue(b, s);
+    if (!b->exec) {
+        hipStream_t cs;
+        if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return GRNS_ERR_HIP;
+        grns_status st = GRNS_OK;
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) st = GRNS_ERR_HIP;
+        if (st == GRNS_OK) st = enqueue(b, cs);
+        hipGraph_t g = nullptr;
+        if (hipStreamEndCapture(cs, &g) != hipSuccess) st = GRNS_ERR_HIP;
+        (void)hipStreamDestroy(cs);
+        if (st != GRNS_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        b->graph = g;
+        if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            drop_graph(b);
+            return GRNS_ERR_HIP;
+        }
+    }
+    return hipGraphLaunch(b->exec, s) == hipSuccess ? GRNS_OK : GRNS_ERR_HIP;
+}
+
+grns_status grns_batch_tune(grns_batch *b, void *stream, int reps, grns_batch_config *best, double *best_ms) {
+    if (!b || reps < 1) return GRNS_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    grns_batch_config keep = b->cfg, top = b->cfg;
+    double top_ms = -1;
+    const int tiles[] = {256, 1024, 4096, 16384};
+    const int bpcs[] = {2, 4, 8};
+    std::vector<int> maps;
+    bool any_elem = false;
+    for (const auto &u : b->user) any_elem |= elementwise(u.op);
+    maps.push_back(0);
+    if (any_elem && thread_kernel_available(b->ctx->n)) maps.push_back(1);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return GRNS_ERR_HIP;
+    grns_status st = GRNS_OK;
+    for (int m : maps) {
+        for (int tl : tiles) {
+            for (int bp : bpcs) {
+                grns_batch_config c{tl, bp, m, keep.use_graph};
+                if ((st = grns_batch_set_config(b, &c)) != GRNS_OK) break;
+                if ((st = grns_batch_launch(b, s)) != GRNS_OK) break;  // warm-up (and graph build)
+                (void)hipEventRecord(e0, s);
+                for (int r = 0; r < reps && st == GRNS_OK; r++) st = grns_batch_launch(b, s);
+                (void)hipEventRecord(e1, s);
+                if (hipEventSynchronize(e1) != hipSuccess) st = GRNS_ERR_HIP;
+                if (st != GRNS_OK) break;
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                double per = ms / reps;
+                if (top_ms < 0 || per < top_ms) {
+                    top_ms = per;
+                    top = c;
+                }
+            }
+            if (st != GRNS_OK) break;
+        }
+        if (st != GRNS_OK) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != GRNS_OK) {
+        grns_batch_set_config(b, &keep);
+        return st;
+    }
+    st = grns_batch_set_config(b, &top);
+    if (best) *best = top;
+    if (best_ms) *best_ms = top_ms;
+    return st;
+}
+
+grns_status grns_batch_destroy(grns_batch *b) {
+    if (!b) return GRNS_ERR_ARG;
+    drop_graph(b);
+    if (b->d_tasks) (void)hipFree(b->d_tasks);
+    if (b->d_counters) (void)hipFree(b->d_counters);
+    delete b;
+    return GRNS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,28 @@
+// grns_internal.hpp -- library-internal declarations shared by the host translation units.
+#pragma once
+
+#include <vector>
+
+#include "../../include/grns.h"
+#include "grns_kernels.hpp"
+
+struct grns_ctx {
+    int device = 0;
+    int n = 0;
+    int log2M = 0;
+    grns::DevCtx h;                // host copy of the device constants
+    grns::DevCtx *d = nullptr;     // device constants
+    int *d_pow2 = nullptr;
+    int *d_mrc = nullptr;
+    int *d_status = nullptr;
+    std::vector<int> pow2;
+    std::vector<int> mrc;
+};
+
+namespace grns {
+inline int groups_for(int n) {
+    int G = 1;
+    while (G < n) G <<= 1;
+    return G;
+}
+}  // namespace grns

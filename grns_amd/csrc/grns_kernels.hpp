@@ -112,23 +112,28 @@ GRNS_D void coop_eval_lanes(const DevCtx *c, int G, int L, const int *__restrict
     }
 }
 
+// One evaluation per lane (number j, valid when j < count); all 64 lanes of the wave call it.
+template <int N, bool FAST>
+GRNS_D void eval_step(const DevCtx *__restrict__ c, const int *__restrict__ x, long long smod, long long snum,
+                      long long j, long long count, interval_t *__restrict__ out, int *__restrict__ status) {
+    bool rare = false;
+    if (j < count) {
+        int xr[N];
+        load_number<N>(x, smod, snum, j, xr);
+        interval_t r;
+        if (eval_hot<N, FAST>(c, xr, r)) store_interval(out + j, r);
+        else rare = true;
+    }
+    unsigned long long mask = __ballot(rare);
+    if (mask) coop_eval_lanes<FAST>(c, Pow2Ceil<N>::v, N, x, smod, snum, j, mask, out, status);
+}
+
 template <int N, bool FAST>
 __global__ __launch_bounds__(BLOCK) void k_eval(const DevCtx *__restrict__ c, const int *__restrict__ x, long long smod,
                                                 long long snum, long long count, interval_t *__restrict__ out,
                                                 int *__restrict__ status) {
-    for (long long base = (long long)blockIdx.x * BLOCK; base < count; base += (long long)gridDim.x * BLOCK) {
-        long long j = base + threadIdx.x;
-        bool rare = false;
-        if (j < count) {
-            int xr[N];
-            load_number<N>(x, smod, snum, j, xr);
-            interval_t r;
-            if (eval_hot<N, FAST>(c, xr, r)) store_interval(out + j, r);
-            else rare = true;
-        }
-        unsigned long long mask = __ballot(rare);
-        if (mask) coop_eval_lanes<FAST>(c, Pow2Ceil<N>::v, N, x, smod, snum, j, mask, out, status);
-    }
+    for (long long base = (long long)blockIdx.x * BLOCK; base < count; base += (long long)gridDim.x * BLOCK)
+        eval_step<N, FAST>(c, x, smod, snum, base + threadIdx.x, count, out, status);
 }
 
 // Any n <= 64: one group per number throughout.
@@ -191,12 +196,12 @@ GRNS_HD double filter_frac(const DevCtx *c, const int (&xr)[N], bool &regular, b
 }
 
 // ---------------------------------------------------------------- comparison
+// One comparison per lane (number j, valid when j < count); all 64 lanes of the wave call it.
 template <int N>
-__global__ __launch_bounds__(BLOCK) void k_cmp(const DevCtx *__restrict__ c, const int *__restrict__ x, long long xsm,
-                                               long long xsn, const int *__restrict__ y, long long ysm, long long ysn,
-                                               long long count, int *__restrict__ sign, int *__restrict__ status) {
-    for (long long base = (long long)blockIdx.x * BLOCK; base < count; base += (long long)gridDim.x * BLOCK) {
-        long long j = base + threadIdx.x;
+GRNS_D void cmp_step(const DevCtx *__restrict__ c, const int *__restrict__ x, long long xsm, long long xsn,
+                     const int *__restrict__ y, long long ysm, long long ysn, long long j, long long count,
+                     int *__restrict__ sign, int *__restrict__ status) {
+    {
         bool rare = false;
         if (j < count) {
             int xr[N], yr[N];
@@ -247,7 +252,15 @@ __global__ __launch_bounds__(BLOCK) void k_cmp(const DevCtx *__restrict__ c, con
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_cmp_group(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_cmp(const DevCtx *__restrict__ c, const int *__restrict__ x, long long xsm,
+                                               long long xsn, const int *__restrict__ y, long long ysm, long long ysn,
+                                               long long count, int *__restrict__ sign, int *__restrict__ status) {
+    for (long long base = (long long)blockIdx.x * BLOCK; base < count; base += (long long)gridDim.x * BLOCK)
+        cmp_step<N>(c, x, xsm, xsn, y, ysm, ysn, base + threadIdx.x, count, sign, status);
+}
+
+static __global__ __launch_bounds__(BLOCK) void k_cmp_group(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
                                                      long long xsm, long long xsn, const int *__restrict__ y,
                                                      long long ysm, long long ysn, long long count,
                                                      int *__restrict__ sign, int *__restrict__ status) {
@@ -400,7 +413,7 @@ __global__ __launch_bounds__(BLOCK) void k_max_partial(const DevCtx *__restrict_
 }
 
 // Stage 2: one block reduces the per-block candidates and writes the winner.
-__global__ __launch_bounds__(BLOCK) void k_max_final(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
+static __global__ __launch_bounds__(BLOCK) void k_max_final(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
                                                      long long smod, long long snum, const MaxCand *__restrict__ part,
                                                      int nparts, int *__restrict__ out, long long *__restrict__ out_idx) {
     __shared__ MaxCand sh[BLOCK];
@@ -564,7 +577,7 @@ GRNS_D int group_scale2pow(const DevCtx *c, const Group &g, int xi, unsigned D) 
     return result;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scale2pow(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
+static __global__ __launch_bounds__(BLOCK) void k_scale2pow(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
                                                      long long smod, long long snum, const unsigned *__restrict__ D,
                                                      long long count, int *__restrict__ y,
                                                      const long long *__restrict__ order) {
@@ -604,66 +617,121 @@ GRNS_D void store_er(er_t *p, const er_t &v) {
     *reinterpret_cast<double2 *>(p) = make_double2(v.frac, __longlong_as_double(v.exp));
 }
 
-// OP: 0 add (src/mpint.cuh:160-207), 1 sub (:215-262), 2 mul (:270-278)
-template <int N, int OP>
-__global__ __launch_bounds__(BLOCK) void k_mpint(const DevCtx *__restrict__ c, long long ld, long long count,
-                                                 const int *__restrict__ xd, const int *__restrict__ xs,
-                                                 const er_t *__restrict__ xe, const int *__restrict__ yd,
-                                                 const int *__restrict__ ys, const er_t *__restrict__ ye,
-                                                 int *__restrict__ rd, int *__restrict__ rs, er_t *__restrict__ re,
-                                                 unsigned char *__restrict__ ovf) {
-    for (long long j = (long long)blockIdx.x * BLOCK + threadIdx.x; j < count; j += (long long)gridDim.x * BLOCK) {
-        int a[N], b[N], r[N];
-        load_number<N>(xd, ld, 1, j, a);
-        load_number<N>(yd, ld, 1, j, b);
-        int sx = xs[j], sy = ys[j];
-        er_t x0 = load_er(xe + 2 * j), x1 = load_er(xe + 2 * j + 1);
-        er_t y0 = load_er(ye + 2 * j), y1 = load_er(ye + 2 * j + 1);
-        er_t r0, r1;
-        int sr;
-        if constexpr (OP == 2) {
-            r0 = er_md_rd(x0, y0, c->unit_upp);
-            r1 = er_md_ru(x1, y1, c->unit_low);
-            int nz = 0;
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-                r[i] = mulmod_recip(a[i], b[i], c->m[i], c->dinv[i]);
-                nz |= r[i];
-            }
-            sr = (nz == 0) ? 0 : (sx ^ sy);
-        } else {
-            if (OP == 1) sy ^= 1;
-            int alpha = 1 - 2 * sx, beta = 1 - 2 * sy;
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-                int res = addmod_recip(alpha * a[i], beta * b[i], c->m[i], c->dinv[i]);
-                r[i] = res < 0 ? res + c->m[i] : res;
-            }
-            x0.frac *= alpha; x1.frac *= alpha;
-            y0.frac *= beta; y1.frac *= beta;
-            er_t ea = sx ? x1 : x0, eb = sy ? y1 : y0;
-            er_t fa = sx ? x0 : x1, fb = sy ? y0 : y1;
-            r0 = er_addsub<false, false>(ea, eb);
-            r1 = er_addsub<false, true>(fa, fb);
-            int minus = r0.frac < 0 && r1.frac < 0;
-            sr = minus;
-            if (minus) {
-#pragma unroll
-                for (int i = 0; i < N; i++) r[i] = r[i] == 0 ? 0 : (c->m[i] - r[i]);
-                er_t tmp = r0;
-                r0.frac = -1 * r1.frac;
-                r0.exp = r1.exp;
-                r1.frac = -1 * tmp.frac;
-                r1.exp = tmp.exp;
-            }
+// mpint operands and results as arrays (digits modulus-major with leading dimension ld).
+struct MpintArgs {
+    const int *xd;
+    const int *xs;
+    const er_t *xe;
+    const int *yd;
+    const int *ys;
+    const er_t *ye;
+    int *rd;
+    int *rs;
+    er_t *re;
+    unsigned char *ovf;
+    long long ld;
+};
+
+// Interval part of mpint_add / mpint_sub (src/mpint.cuh:160-262, OP 0/1) and mpint_mul (:270-278,
+// OP 2).  For add/sub, `minus` tells whether the digits are to be negated.
+template <int OP>
+GRNS_D void mpint_bounds(const DevCtx *c, int sx, int sy, er_t x0, er_t x1, er_t y0, er_t y1, er_t &r0, er_t &r1,
+                         int &minus) {
+    if constexpr (OP == 2) {
+        r0 = er_md_rd(x0, y0, c->unit_upp);
+        r1 = er_md_ru(x1, y1, c->unit_low);
+        minus = 0;
+    } else {
+        int alpha = 1 - 2 * sx, beta = 1 - 2 * sy;
+        x0.frac *= alpha; x1.frac *= alpha;
+        y0.frac *= beta; y1.frac *= beta;
+        er_t ea = sx ? x1 : x0, eb = sy ? y1 : y0;
+        er_t fa = sx ? x0 : x1, fb = sy ? y0 : y1;
+        r0 = er_addsub<false, false>(ea, eb);
+        r1 = er_addsub<false, true>(fa, fb);
+        minus = r0.frac < 0 && r1.frac < 0;
+        if (minus) {
+            er_t tmp = r0;
+            r0.frac = -1 * r1.frac;
+            r0.exp = r1.exp;
+            r1.frac = -1 * tmp.frac;
+            r1.exp = tmp.exp;
         }
-#pragma unroll
-        for (int i = 0; i < N; i++) __builtin_nontemporal_store(r[i], rd + (long long)i * ld + j);
-        rs[j] = sr;
-        store_er(re + 2 * j, r0);
-        store_er(re + 2 * j + 1, r1);
-        if (ovf) ovf[j] = check_overflow(c, r0, r1);
     }
+}
+
+// Digit of modulus i (reciprocal reductions of the CPU path).
+template <int OP>
+GRNS_D int mpint_digit(int a, int b, int sx, int sy, int m, double dinv) {
+    if constexpr (OP == 2) {
+        return mulmod_recip(a, b, m, dinv);
+    } else {
+        int res = addmod_recip((1 - 2 * sx) * a, (1 - 2 * sy) * b, m, dinv);
+        return res < 0 ? res + m : res;
+    }
+}
+
+// Thread per number: element j (j < count).  OP: 0 add, 1 sub, 2 mul.
+template <int N, int OP>
+GRNS_D void mpint_step(const DevCtx *__restrict__ c, const MpintArgs &A, long long j) {
+    int a[N], b[N], r[N];
+    load_number<N>(A.xd, A.ld, 1, j, a);
+    load_number<N>(A.yd, A.ld, 1, j, b);
+    int sx = A.xs[j], sy = A.ys[j];
+    if (OP == 1) sy ^= 1;
+    er_t x0 = load_er(A.xe + 2 * j), x1 = load_er(A.xe + 2 * j + 1);
+    er_t y0 = load_er(A.ye + 2 * j), y1 = load_er(A.ye + 2 * j + 1);
+    er_t r0, r1;
+    int minus;
+    mpint_bounds<OP>(c, sx, sy, x0, x1, y0, y1, r0, r1, minus);
+    int nz = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        r[i] = mpint_digit<OP>(a[i], b[i], sx, sy, c->m[i], c->dinv[i]);
+        if (OP != 2 && minus) r[i] = r[i] == 0 ? 0 : (c->m[i] - r[i]);
+        nz |= r[i];
+    }
+    int sr = (OP == 2) ? ((nz == 0) ? 0 : (sx ^ sy)) : minus;
+#pragma unroll
+    for (int i = 0; i < N; i++) __builtin_nontemporal_store(r[i], A.rd + (long long)i * A.ld + j);
+    A.rs[j] = sr;
+    store_er(A.re + 2 * j, r0);
+    store_er(A.re + 2 * j + 1, r1);
+    if (A.ovf) A.ovf[j] = check_overflow(c, r0, r1);
+}
+
+// Group per number: lane gl handles digit gl; the bounds are formed on every lane.
+template <int OP>
+GRNS_D void group_mpint(const DevCtx *__restrict__ c, const Group &g, const MpintArgs &A, long long j) {
+    const bool act = g.gl < g.L;
+    long long off = (long long)g.gl * A.ld + j;
+    int a = act ? A.xd[off] : 0, b = act ? A.yd[off] : 0;
+    int sx = A.xs[j], sy = A.ys[j];
+    if (OP == 1) sy ^= 1;
+    er_t r0, r1;
+    int minus;
+    mpint_bounds<OP>(c, sx, sy, load_er(A.xe + 2 * j), load_er(A.xe + 2 * j + 1), load_er(A.ye + 2 * j),
+                     load_er(A.ye + 2 * j + 1), r0, r1, minus);
+    int r = 0;
+    if (act) {
+        int m = c->m[g.gl];
+        r = mpint_digit<OP>(a, b, sx, sy, m, c->dinv[g.gl]);
+        if (OP != 2 && minus) r = r == 0 ? 0 : (m - r);
+        A.rd[off] = r;
+    }
+    int sr = (OP == 2) ? (group_all(g, !act || r == 0) ? 0 : (sx ^ sy)) : minus;
+    if (g.gl == 0) {
+        A.rs[j] = sr;
+        store_er(A.re + 2 * j, r0);
+        store_er(A.re + 2 * j + 1, r1);
+        if (A.ovf) A.ovf[j] = check_overflow(c, r0, r1);
+    }
+}
+
+template <int N, int OP>
+__global__ __launch_bounds__(BLOCK) void k_mpint(const DevCtx *__restrict__ c, MpintArgs A, long long count) {
+    for (long long j = (long long)blockIdx.x * BLOCK + threadIdx.x; j < count; j += (long long)gridDim.x * BLOCK)
+        mpint_step<N, OP>(c, A, j);
 }
 
 }  // namespace grns

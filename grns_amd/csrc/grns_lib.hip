@@ -12,22 +12,10 @@
 
 #include "../../include/grns.h"
 #include "../../include/grns_debug.h"
-#include "grns_kernels.hpp"
+#include "grns_internal.hpp"
 
 using namespace grns;
 
-struct grns_ctx {
-    int device = 0;
-    int n = 0;
-    int log2M = 0;
-    DevCtx h;                      // host copy of the device constants
-    DevCtx *d = nullptr;           // device constants
-    int *d_pow2 = nullptr;
-    int *d_mrc = nullptr;
-    int *d_status = nullptr;
-    std::vector<int> pow2;
-    std::vector<int> mrc;
-};
 
 namespace {
 
@@ -171,11 +159,6 @@ grns_status build_constants(grns_ctx *c, const int *moduli, int n) {
     return GRNS_OK;
 }
 
-int groups_for(int n) {
-    int G = 1;
-    while (G < n) G <<= 1;
-    return G;
-}
 
 hipStream_t S(void *s) { return (hipStream_t)s; }
 
@@ -449,14 +432,16 @@ static grns_status mpint_launch(grns_ctx *c, int op, const grns_mpint_array *x, 
     DeviceGuard guard(c->device);
     dim3 grid((unsigned)grid_elems(count)), blk(BLOCK);
     long long cnt = (long long)count;
-    const er_t *xe = reinterpret_cast<const er_t *>(x->eval);
-    const er_t *ye = reinterpret_cast<const er_t *>(y->eval);
-    er_t *ze = reinterpret_cast<er_t *>(z->eval);
-#define GRNS_MP_CASE(NN)                                                                                                  \
-    case NN:                                                                                                              \
-        if (op == 0) hipLaunchKernelGGL((k_mpint<NN, 0>), grid, blk, 0, S(stream), c->d, x->ld, cnt, x->digits, x->sign, xe, y->digits, y->sign, ye, z->digits, z->sign, ze, ovf); \
-        else if (op == 1) hipLaunchKernelGGL((k_mpint<NN, 1>), grid, blk, 0, S(stream), c->d, x->ld, cnt, x->digits, x->sign, xe, y->digits, y->sign, ye, z->digits, z->sign, ze, ovf); \
-        else hipLaunchKernelGGL((k_mpint<NN, 2>), grid, blk, 0, S(stream), c->d, x->ld, cnt, x->digits, x->sign, xe, y->digits, y->sign, ye, z->digits, z->sign, ze, ovf); \
+    MpintArgs A;
+    A.xd = x->digits; A.xs = x->sign; A.xe = reinterpret_cast<const er_t *>(x->eval);
+    A.yd = y->digits; A.ys = y->sign; A.ye = reinterpret_cast<const er_t *>(y->eval);
+    A.rd = z->digits; A.rs = z->sign; A.re = reinterpret_cast<er_t *>(z->eval);
+    A.ovf = ovf; A.ld = x->ld;
+#define GRNS_MP_CASE(NN)                                                                               \
+    case NN:                                                                                           \
+        if (op == 0) hipLaunchKernelGGL((k_mpint<NN, 0>), grid, blk, 0, S(stream), c->d, A, cnt);      \
+        else if (op == 1) hipLaunchKernelGGL((k_mpint<NN, 1>), grid, blk, 0, S(stream), c->d, A, cnt); \
+        else hipLaunchKernelGGL((k_mpint<NN, 2>), grid, blk, 0, S(stream), c->d, A, cnt);              \
         return launch_check();
     switch (c->n) {
         GRNS_MP_CASE(4)

@@ -110,6 +110,58 @@ grns_status grns_mpint_sub(grns_ctx *ctx, const grns_mpint_array *x, const grns_
 grns_status grns_mpint_mul(grns_ctx *ctx, const grns_mpint_array *x, const grns_mpint_array *y,
                            const grns_mpint_array *z, size_t count, unsigned char *overflow, void *stream);
 
+/* ---------------------------------------------------------------- batched form (no reference
+ * counterpart): many arrays of uneven length and several operations of one moduli set fused into
+ * one persistent, work-stealing launch that can be captured in a hipGraph.  Each task is one of
+ * the element-wise entry points above applied to its own arrays; results are identical to the
+ * per-call entry points. */
+enum {
+    GRNS_OP_EVAL = 0,       /* grns_eval, fast = 0 : a -> out0 (grns_interval[count])        */
+    GRNS_OP_EVAL_FAST = 1,  /* grns_eval, fast = 1                                           */
+    GRNS_OP_CMP = 2,        /* grns_cmp            : a, b -> out0 (int[count])               */
+    GRNS_OP_MPINT_ADD = 3,  /* grns_mpint_add      : mx, my -> mz (+ overflow)               */
+    GRNS_OP_MPINT_SUB = 4,
+    GRNS_OP_MPINT_MUL = 5,
+    GRNS_OP_DIV = 6,        /* grns_div, fast = 0  : a (x), b (d) -> out0 (q), out1 (r); a, b, q, r share a's strides */
+    GRNS_OP_DIV_FAST = 7,
+    GRNS_OP_SCALE2POW = 8   /* grns_scale2pow      : a, D -> out0 (same strides as a)        */
+};
+
+typedef struct {
+    int op;
+    size_t count;
+    const int *a;
+    long long a_mod_stride, a_num_stride;
+    const int *b;
+    long long b_mod_stride, b_num_stride;
+    void *out0;
+    void *out1;
+    const unsigned *D;
+    grns_mpint_array mx, my, mz;   /* mpint operands (digits ld >= count) */
+    unsigned char *overflow;       /* mpint overflow flags, optional       */
+} grns_batch_task;
+
+/* Launch shape: numbers per work tile, persistent blocks per CU, and the mapping of numbers to
+ * lanes (threads_per_number = 1: one lane per number; 0: one group of 2^ceil(log2 n) lanes per
+ * number).  Division and scaling always use groups. */
+typedef struct {
+    int tile;
+    int blocks_per_cu;
+    int threads_per_number;
+    int use_graph;
+} grns_batch_config;
+
+typedef struct grns_batch grns_batch;
+grns_status grns_batch_create(grns_ctx *ctx, const grns_batch_task *tasks, int ntasks, grns_batch **out);
+grns_status grns_batch_set_config(grns_batch *b, const grns_batch_config *cfg);
+grns_status grns_batch_get_config(const grns_batch *b, grns_batch_config *cfg);
+/* Times the candidate launch shapes on this batch (reps launches each) and keeps the fastest. */
+grns_status grns_batch_tune(grns_batch *b, void *stream, int reps, grns_batch_config *best, double *best_ms);
+/* Runs every task of the batch (one persistent launch per mapping; replayed from a hipGraph when
+ * use_graph is set). */
+grns_status grns_batch_launch(grns_batch *b, void *stream);
+grns_status grns_batch_destroy(grns_batch *b);
+
 /* Version string of the library. */
 const char *grns_version(void);
 
