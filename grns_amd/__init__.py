@@ -69,7 +69,8 @@ EXPORTS = [
     "grns_ctx_tables", "grns_ctx_status", "grns_eval", "grns_cmp", "grns_div", "grns_scale2pow",
     "grns_max_workspace_size", "grns_max", "grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul",
     "grns_batch_create", "grns_batch_set_config", "grns_batch_get_config", "grns_batch_tune", "grns_batch_launch",
-    "grns_batch_destroy",
+    "grns_batch_destroy", "grns_serve_create", "grns_serve_submit", "grns_serve_poll", "grns_serve_wait",
+    "grns_serve_drain", "grns_serve_stats_get", "grns_serve_destroy",
 ]
 DEBUG_EXPORTS = [
     "grns_debug_eval_hot", "grns_debug_filter", "grns_debug_ceil_log2", "grns_debug_mulmod_recip_d",
@@ -363,6 +364,76 @@ class Batch:
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             lib().grns_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("slots", C.c_int), ("max_batch", C.c_int), ("tile", C.c_int),
+                ("blocks_per_cu", C.c_int)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong), ("rejected", C.c_longlong),
+                ("launches", C.c_longlong), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_ms", C.c_double), ("max_ms", C.c_double)]
+
+
+OPS["max"] = 9
+
+
+class Server:
+    """Request loop on one device (grns_serve_*): continuous batching with HBM-byte admission."""
+
+    def __init__(self, device=0, hbm_budget=8 << 30, slots=4, max_batch=64, tile=1024, blocks_per_cu=4):
+        L = lib()
+        P, I = C.c_void_p, C.c_int
+        L.grns_serve_create.argtypes = [I, C.POINTER(ServeConfig), C.POINTER(P)]
+        L.grns_serve_submit.argtypes = [P, P, C.POINTER(BatchTask), I, C.POINTER(C.c_longlong)]
+        L.grns_serve_poll.argtypes = [P, C.c_longlong, C.POINTER(I), C.POINTER(C.c_double)]
+        L.grns_serve_wait.argtypes = [P, C.c_longlong, C.POINTER(C.c_double)]
+        L.grns_serve_drain.argtypes = [P]
+        L.grns_serve_stats_get.argtypes = [P, C.POINTER(ServeStats)]
+        L.grns_serve_destroy.argtypes = [P]
+        cfg = ServeConfig(hbm_budget, slots, max_batch, tile, blocks_per_cu)
+        h = C.c_void_p()
+        _check(L.grns_serve_create(device, C.byref(cfg), C.byref(h)), "grns_serve_create")
+        self._h = h
+
+    def submit(self, ctx: Context, tasks):
+        arr = (BatchTask * len(tasks))(*tasks)
+        t = C.c_longlong()
+        rc = lib().grns_serve_submit(self._h, ctx.handle, arr, len(tasks), C.byref(t))
+        if rc != GRNS_OK:
+            raise GrnsError(rc, "grns_serve_submit")
+        return t.value
+
+    def poll(self, ticket):
+        st, lat = C.c_int(), C.c_double()
+        rc = lib().grns_serve_poll(self._h, ticket, C.byref(st), C.byref(lat))
+        return st.value, lat.value, rc
+
+    def wait(self, ticket):
+        lat = C.c_double()
+        _check(lib().grns_serve_wait(self._h, ticket, C.byref(lat)), "grns_serve_wait")
+        return lat.value
+
+    def drain(self):
+        _check(lib().grns_serve_drain(self._h), "grns_serve_drain")
+
+    def stats(self):
+        s = ServeStats()
+        _check(lib().grns_serve_stats_get(self._h, C.byref(s)), "grns_serve_stats_get")
+        return {f: getattr(s, f) for f, _ in ServeStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().grns_serve_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -171,20 +171,6 @@ __global__ void k_batch_reset(unsigned long long *counters) {
 
 }  // namespace grns
 
-struct grns_batch {
-    grns_ctx *ctx = nullptr;
-    std::vector<grns_batch_task> user;
-    std::vector<BatchTask> thr, grp;   // task lists of the two mappings, tile0 filled in
-    long long thr_tiles = 0, grp_tiles = 0;
-    BatchTask *d_tasks = nullptr;
-    unsigned long long *d_counters = nullptr;
-    int cap = 0;
-    grns_batch_config cfg{1024, 4, 1, 0};
-    int cus = 256;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-};
-
 namespace {
 
 bool thread_kernel_available(int n) { return n == 4 || n == 8 || n == 16 || n == 32 || n == 64; }
@@ -200,54 +186,56 @@ BatchTask to_dev(const grns_batch_task &u) {
     t.b = u.b; t.bsm = u.b_mod_stride; t.bsn = u.b_num_stride;
     t.o0 = u.out0; t.o1 = u.out1; t.D = u.D;
     t.mp.xd = u.mx.digits; t.mp.xs = u.mx.sign; t.mp.xe = reinterpret_cast<const er_t *>(u.mx.eval);
-    t.mp.yd = u.my.digits; t.mp.ys = u.my.sign; t.mp.ye = reinterpret_cast<const er_t *>(u.my.eval);
-    t.mp.rd = u.mz.digits; t.mp.rs = u.mz.sign; t.mp.re = reinterpret_cast<er_t *>(u.mz.eval);
+    t.mp.yd = u.my.digits; t.mp.ys = u.my.sign; t.mp.re = reinterpret_cast<er_t *>(u.mz.eval);
+    t.mp.ye = reinterpret_cast<const er_t *>(u.my.eval);
+    t.mp.rd = u.mz.digits; t.mp.rs = u.mz.sign;
     t.mp.ovf = u.overflow; t.mp.ld = u.mx.ld;
     return t;
 }
 
-void drop_graph(grns_batch *b) {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
-}
+// Tasks split by mapping (thread-per-number first, then group-per-number), with the tile prefix
+// of each list filled in.
+struct TaskPlan {
+    std::vector<BatchTask> all;
+    int nthr = 0, ngrp = 0;
+    long long thr_tiles = 0, grp_tiles = 0;
+};
 
-grns_status rebuild(grns_batch *b) {
-    drop_graph(b);
-    b->thr.clear();
-    b->grp.clear();
-    const bool thread_ok = thread_kernel_available(b->ctx->n) && b->cfg.threads_per_number == 1;
-    long long tile = b->cfg.tile;
-    for (const auto &u : b->user) {
-        if (u.count == 0) continue;
-        BatchTask t = to_dev(u);
-        auto &lst = (thread_ok && elementwise(u.op)) ? b->thr : b->grp;
-        lst.push_back(t);
+void make_plan(const grns_batch_task *u, int n, int ctx_n, const grns_batch_config &cfg, TaskPlan &P) {
+    P.all.clear();
+    std::vector<BatchTask> grp;
+    const bool thread_ok = thread_kernel_available(ctx_n) && cfg.threads_per_number == 1;
+    const long long tile = cfg.tile;
+    P.thr_tiles = P.grp_tiles = 0;
+    for (int i = 0; i < n; i++) {
+        if (u[i].count == 0 || u[i].op == GRNS_OP_MAX) continue;
+        BatchTask t = to_dev(u[i]);
+        if (thread_ok && elementwise(u[i].op)) {
+            t.tile0 = P.thr_tiles;
+            P.thr_tiles += (t.count + tile - 1) / tile;
+            P.all.push_back(t);
+        } else {
+            t.tile0 = P.grp_tiles;
+            P.grp_tiles += (t.count + tile - 1) / tile;
+            grp.push_back(t);
+        }
     }
-    b->thr_tiles = 0;
-    for (auto &t : b->thr) { t.tile0 = b->thr_tiles; b->thr_tiles += (t.count + tile - 1) / tile; }
-    b->grp_tiles = 0;
-    for (auto &t : b->grp) { t.tile0 = b->grp_tiles; b->grp_tiles += (t.count + tile - 1) / tile; }
-    std::vector<BatchTask> all(b->thr);
-    all.insert(all.end(), b->grp.begin(), b->grp.end());
-    if (!all.empty() &&
-        hipMemcpy(b->d_tasks, all.data(), sizeof(BatchTask) * all.size(), hipMemcpyHostToDevice) != hipSuccess)
-        return GRNS_ERR_HIP;
-    return GRNS_OK;
+    P.nthr = (int)P.all.size();
+    P.ngrp = (int)grp.size();
+    P.all.insert(P.all.end(), grp.begin(), grp.end());
 }
 
-grns_status enqueue(grns_batch *b, hipStream_t s) {
-    grns_ctx *c = b->ctx;
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, b->d_counters);
-    dim3 grid((unsigned)(b->cus * b->cfg.blocks_per_cu)), blk(BLOCK);
-    int nthr = (int)b->thr.size(), ngrp = (int)b->grp.size();
-    if (nthr) {
+// Counter reset plus one persistent launch per non-empty list; d_tasks must hold P.all.
+grns_status enqueue_plan(grns_ctx *c, const TaskPlan &P, const BatchTask *d_tasks, unsigned long long *d_counters,
+                         const grns_batch_config &cfg, int cus, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, d_counters);
+    dim3 grid((unsigned)(cus * cfg.blocks_per_cu)), blk(BLOCK);
+    if (P.nthr) {
         switch (c->n) {
-#define GRNS_BT(NN)                                                                                                  \
-    case NN:                                                                                                         \
-        hipLaunchKernelGGL((k_batch_thread<NN>), grid, blk, 0, s, c->d, b->d_tasks, nthr, b->thr_tiles, b->cfg.tile, \
-                           b->d_counters, c->d_status);                                                              \
+#define GRNS_BT(NN)                                                                                          \
+    case NN:                                                                                                 \
+        hipLaunchKernelGGL((k_batch_thread<NN>), grid, blk, 0, s, c->d, d_tasks, P.nthr, P.thr_tiles, cfg.tile, \
+                           d_counters, c->d_status);                                                         \
         break;
             GRNS_BT(4)
             GRNS_BT(8)
@@ -259,11 +247,48 @@ grns_status enqueue(grns_batch *b, hipStream_t s) {
                 return GRNS_ERR_UNSUPPORTED;
         }
     }
-    if (ngrp) {
-        hipLaunchKernelGGL(k_batch_group, grid, blk, 0, s, c->d, groups_for(c->n), b->d_tasks + nthr, ngrp,
-                           b->grp_tiles, b->cfg.tile, b->d_counters + 1, c->d_status);
+    if (P.ngrp) {
+        hipLaunchKernelGGL(k_batch_group, grid, blk, 0, s, c->d, groups_for(c->n), d_tasks + P.nthr, P.ngrp,
+                           P.grp_tiles, cfg.tile, d_counters + 1, c->d_status);
     }
     return hipGetLastError() == hipSuccess ? GRNS_OK : GRNS_ERR_HIP;
+}
+
+}  // namespace
+
+struct grns_batch {
+    grns_ctx *ctx = nullptr;
+    std::vector<grns_batch_task> user;
+    TaskPlan plan;
+    BatchTask *d_tasks = nullptr;
+    unsigned long long *d_counters = nullptr;
+    int cap = 0;
+    grns_batch_config cfg{1024, 4, 1, 0};
+    int cus = 256;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void drop_graph(grns_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+grns_status rebuild(grns_batch *b) {
+    drop_graph(b);
+    make_plan(b->user.data(), (int)b->user.size(), b->ctx->n, b->cfg, b->plan);
+    if (!b->plan.all.empty() && hipMemcpy(b->d_tasks, b->plan.all.data(), sizeof(BatchTask) * b->plan.all.size(),
+                                          hipMemcpyHostToDevice) != hipSuccess)
+        return GRNS_ERR_HIP;
+    return GRNS_OK;
+}
+
+grns_status enqueue(grns_batch *b, hipStream_t s) {
+    return enqueue_plan(b->ctx, b->plan, b->d_tasks, b->d_counters, b->cfg, b->cus, s);
 }
 
 }  // namespace
@@ -274,7 +299,7 @@ grns_status grns_batch_create(grns_ctx *ctx, const grns_batch_task *tasks, int n
     if (!ctx || !out || ntasks < 0 || (ntasks && !tasks) || ctx->device < 0 || ctx->n > 64) return GRNS_ERR_ARG;
     for (int i = 0; i < ntasks; i++) {
         const grns_batch_task &u = tasks[i];
-        if (u.op < GRNS_OP_EVAL || u.op > GRNS_OP_SCALE2POW) return GRNS_ERR_ARG;
+        if (u.op < GRNS_OP_EVAL || u.op > GRNS_OP_SCALE2POW) return GRNS_ERR_ARG;  // arg-max: use grns_max
         if (u.count == 0) continue;
         bool mp = u.op >= GRNS_OP_MPINT_ADD && u.op <= GRNS_OP_MPINT_MUL;
         if (mp) {
@@ -334,7 +359,7 @@ grns_status grns_batch_launch(grns_batch *b, void *stream) {
     if (!b) return GRNS_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
     (void)hipSetDevice(b->ctx->device);
-    if (b->thr.empty() && b->grp.empty()) return GRNS_OK;
+    if (b->plan.all.empty()) return GRNS_OK;
     if (!b->cfg.use_graph) return enqueue(b, s);
     if (!b->exec) {
         hipStream_t cs;
@@ -414,6 +439,334 @@ grns_status grns_batch_destroy(grns_batch *b) {
     if (b->d_tasks) (void)hipFree(b->d_tasks);
     if (b->d_counters) (void)hipFree(b->d_counters);
     delete b;
+    return GRNS_OK;
+}
+
+}  // extern "C"
+
+// ============================================================================ serving form
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+size_t task_bytes(const grns_batch_task &t, int n) {
+    const size_t c = t.count, nn = (size_t)n;
+    switch (t.op) {
+        case GRNS_OP_EVAL:
+        case GRNS_OP_EVAL_FAST: return c * (4 * nn + 32);
+        case GRNS_OP_CMP: return c * (8 * nn + 4);
+        case GRNS_OP_MPINT_ADD:
+        case GRNS_OP_MPINT_SUB:
+        case GRNS_OP_MPINT_MUL: return c * 3 * (4 * nn + 36);
+        case GRNS_OP_DIV:
+        case GRNS_OP_DIV_FAST: return c * 16 * nn;
+        case GRNS_OP_SCALE2POW: return c * (8 * nn + 4);
+        case GRNS_OP_MAX: return c * 4 * nn;
+        default: return 0;
+    }
+}
+
+struct Request {
+    long long ticket;
+    grns_ctx *ctx;
+    std::vector<grns_batch_task> tasks;
+    size_t bytes;
+    Clock::time_point t0;
+};
+
+struct TicketRec {
+    int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
+    double latency_ms = 0;
+    grns_status result = GRNS_OK;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    BatchTask *d_tasks = nullptr;
+    BatchTask *h_stage = nullptr;
+    unsigned long long *d_counters = nullptr;
+    void *max_ws = nullptr;
+    size_t max_ws_bytes = 0;
+    int cap = 0;
+    bool busy = false;
+    std::vector<Request> reqs;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct grns_server {
+    int device = 0;
+    grns_serve_config cfg{};
+    int cus = 256;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Request> pending;
+    std::deque<TicketRec> tickets;
+    size_t budget_free = 0;
+    bool stop = false;
+    std::thread worker;
+    std::vector<double> lat;
+    long long submitted = 0, completed = 0, rejected = 0, launches = 0;
+};
+
+namespace {
+
+grns_status launch_group(grns_server *S, Slot &sl) {
+    std::vector<grns_batch_task> flat;
+    grns_ctx *c = sl.reqs.front().ctx;
+    for (auto &r : sl.reqs)
+        for (auto &t : r.tasks) flat.push_back(t);
+    grns_batch_config bc{S->cfg.tile, S->cfg.blocks_per_cu, 1, 0};
+    TaskPlan plan;
+    make_plan(flat.data(), (int)flat.size(), c->n, bc, plan);
+    grns_status st = GRNS_OK;
+    if (!plan.all.empty()) {
+        std::memcpy(sl.h_stage, plan.all.data(), sizeof(BatchTask) * plan.all.size());
+        if (hipMemcpyAsync(sl.d_tasks, sl.h_stage, sizeof(BatchTask) * plan.all.size(), hipMemcpyHostToDevice,
+                           sl.stream) != hipSuccess)
+            st = GRNS_ERR_HIP;
+        if (st == GRNS_OK) st = enqueue_plan(c, plan, sl.d_tasks, sl.d_counters, bc, S->cus, sl.stream);
+    }
+    for (auto &t : flat) {
+        if (st != GRNS_OK) break;
+        if (t.op != GRNS_OP_MAX || t.count == 0) continue;
+        st = grns_max(c, t.a, t.a_mod_stride, t.a_num_stride, t.count, (int *)t.out0, (long long *)t.out1, sl.max_ws,
+                      sl.max_ws_bytes, sl.stream);
+    }
+    if (hipEventRecord(sl.done, sl.stream) != hipSuccess && st == GRNS_OK) st = GRNS_ERR_HIP;
+    return st;
+}
+
+void serve_loop(grns_server *S) {
+    (void)hipSetDevice(S->device);
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (;;) {
+        bool progress = false;
+        // completions
+        for (auto &sl : S->slots) {
+            if (!sl.busy) continue;
+            lk.unlock();
+            hipError_t q = hipEventQuery(sl.done);
+            lk.lock();
+            if (q == hipErrorNotReady) continue;
+            auto now = Clock::now();
+            for (auto &r : sl.reqs) {
+                TicketRec &t = S->tickets[(size_t)r.ticket];
+                t.latency_ms = std::chrono::duration<double, std::milli>(now - r.t0).count();
+                t.result = (q == hipSuccess) ? GRNS_OK : GRNS_ERR_HIP;
+                t.state = (q == hipSuccess) ? 2 : 3;
+                if (q == hipSuccess) S->lat.push_back(t.latency_ms);
+                S->completed++;
+            }
+            S->budget_free += sl.bytes;
+            sl.busy = false;
+            sl.reqs.clear();
+            sl.bytes = 0;
+            progress = true;
+        }
+        if (progress) S->cv.notify_all();
+        // admission: arrival order; a launch fuses pending requests of the head's moduli set that
+        // fit the remaining budget
+        for (auto &sl : S->slots) {
+            if (sl.busy || S->pending.empty()) continue;
+            if (S->pending.front().bytes > S->budget_free) break;
+            grns_ctx *cx = S->pending.front().ctx;
+            size_t bytes = 0;
+            int ntasks = 0;
+            for (auto it = S->pending.begin(); it != S->pending.end() && (int)sl.reqs.size() < S->cfg.max_batch;) {
+                bool fits = it->ctx == cx && bytes + it->bytes <= S->budget_free &&
+                            ntasks + (int)it->tasks.size() <= sl.cap;
+                if (fits) {
+                    bytes += it->bytes;
+                    ntasks += (int)it->tasks.size();
+                    S->tickets[(size_t)it->ticket].state = 1;
+                    sl.reqs.push_back(std::move(*it));
+                    it = S->pending.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            S->budget_free -= bytes;
+            sl.bytes = bytes;
+            sl.busy = true;
+            S->launches++;
+            lk.unlock();
+            grns_status st = launch_group(S, sl);
+            lk.lock();
+            if (st != GRNS_OK) {
+                for (auto &r : sl.reqs) {
+                    TicketRec &t = S->tickets[(size_t)r.ticket];
+                    t.state = 3;
+                    t.result = st;
+                    S->completed++;
+                }
+                (void)hipStreamSynchronize(sl.stream);
+                S->budget_free += sl.bytes;
+                sl.busy = false;
+                sl.reqs.clear();
+                sl.bytes = 0;
+                S->cv.notify_all();
+            }
+            progress = true;
+        }
+        bool idle = S->pending.empty();
+        for (auto &sl : S->slots) idle = idle && !sl.busy;
+        if (S->stop && idle) break;
+        if (!progress) {
+            if (idle) S->cv.wait(lk, [S] { return S->stop || !S->pending.empty(); });
+            else S->cv.wait_for(lk, std::chrono::microseconds(20));
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+grns_status grns_serve_create(int device, const grns_serve_config *cfg, grns_server **out) {
+    if (!cfg || !out || cfg->slots < 1 || cfg->slots > 16 || cfg->max_batch < 1 || cfg->hbm_budget == 0 ||
+        cfg->tile < 1 || cfg->blocks_per_cu < 1 || device < 0)
+        return GRNS_ERR_ARG;
+    grns_server *S = new (std::nothrow) grns_server();
+    if (!S) return GRNS_ERR_OOM;
+    S->device = device;
+    S->cfg = *cfg;
+    S->budget_free = cfg->hbm_budget;
+    (void)hipSetDevice(device);
+    if (hipDeviceGetAttribute(&S->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) S->cus = 256;
+    S->slots.resize((size_t)cfg->slots);
+    for (auto &sl : S->slots) {
+        sl.cap = 8 * cfg->max_batch;
+        sl.max_ws_bytes = 64 * 1024;
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
+            hipMalloc(&sl.d_tasks, sizeof(BatchTask) * sl.cap) != hipSuccess ||
+            hipHostMalloc(&sl.h_stage, sizeof(BatchTask) * sl.cap) != hipSuccess ||
+            hipMalloc(&sl.d_counters, 2 * sizeof(unsigned long long)) != hipSuccess ||
+            hipMalloc(&sl.max_ws, sl.max_ws_bytes) != hipSuccess) {
+            grns_serve_destroy(S);
+            return GRNS_ERR_OOM;
+        }
+    }
+    S->worker = std::thread(serve_loop, S);
+    *out = S;
+    return GRNS_OK;
+}
+
+grns_status grns_serve_submit(grns_server *S, grns_ctx *ctx, const grns_batch_task *tasks, int ntasks,
+                              long long *ticket) {
+    if (!S || !ctx || !tasks || ntasks < 1 || !ticket || ctx->device != S->device || ctx->n > 64) return GRNS_ERR_ARG;
+    Request r;
+    r.ctx = ctx;
+    r.bytes = 0;
+    for (int i = 0; i < ntasks; i++) {
+        const grns_batch_task &u = tasks[i];
+        if (u.op < GRNS_OP_EVAL || u.op > GRNS_OP_MAX) return GRNS_ERR_ARG;
+        if (u.op == GRNS_OP_MAX && u.count > 0 && (!u.a || !u.out0 || !u.out1)) return GRNS_ERR_ARG;
+        r.bytes += task_bytes(u, ctx->n);
+    }
+    if (ntasks > 8 * S->cfg.max_batch) return GRNS_ERR_ARG;
+    r.tasks.assign(tasks, tasks + ntasks);
+    std::lock_guard<std::mutex> g(S->mu);
+    r.ticket = (long long)S->tickets.size();
+    S->tickets.emplace_back();
+    *ticket = r.ticket;
+    S->submitted++;
+    if (r.bytes > S->cfg.hbm_budget) {
+        S->tickets.back().state = 3;
+        S->tickets.back().result = GRNS_ERR_ADMISSION;
+        S->rejected++;
+        return GRNS_ERR_ADMISSION;
+    }
+    r.t0 = Clock::now();
+    S->pending.push_back(std::move(r));
+    S->cv.notify_all();
+    return GRNS_OK;
+}
+
+grns_status grns_serve_poll(grns_server *S, long long ticket, int *state, double *latency_ms) {
+    if (!S || !state) return GRNS_ERR_ARG;
+    std::lock_guard<std::mutex> g(S->mu);
+    if (ticket < 0 || ticket >= (long long)S->tickets.size()) return GRNS_ERR_ARG;
+    const TicketRec &t = S->tickets[(size_t)ticket];
+    *state = t.state;
+    if (latency_ms) *latency_ms = t.latency_ms;
+    return t.state == 3 ? t.result : GRNS_OK;
+}
+
+grns_status grns_serve_wait(grns_server *S, long long ticket, double *latency_ms) {
+    if (!S) return GRNS_ERR_ARG;
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (ticket < 0 || ticket >= (long long)S->tickets.size()) return GRNS_ERR_ARG;
+    S->cv.wait(lk, [&] { return S->tickets[(size_t)ticket].state >= 2; });
+    const TicketRec &t = S->tickets[(size_t)ticket];
+    if (latency_ms) *latency_ms = t.latency_ms;
+    return t.state == 2 ? GRNS_OK : t.result;
+}
+
+grns_status grns_serve_drain(grns_server *S) {
+    if (!S) return GRNS_ERR_ARG;
+    std::unique_lock<std::mutex> lk(S->mu);
+    S->cv.wait(lk, [S] {
+        if (!S->pending.empty()) return false;
+        for (auto &sl : S->slots)
+            if (sl.busy) return false;
+        return true;
+    });
+    return GRNS_OK;
+}
+
+grns_status grns_serve_stats_get(grns_server *S, grns_serve_stats *o) {
+    if (!S || !o) return GRNS_ERR_ARG;
+    std::lock_guard<std::mutex> g(S->mu);
+    o->submitted = S->submitted;
+    o->completed = S->completed;
+    o->rejected = S->rejected;
+    o->launches = S->launches;
+    std::vector<double> v(S->lat);
+    o->p50_ms = o->p99_ms = o->mean_ms = o->max_ms = 0;
+    if (!v.empty()) {
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double p) { return v[std::min(v.size() - 1, (size_t)(p * (double)(v.size() - 1) + 0.5))]; };
+        o->p50_ms = pct(0.50);
+        o->p99_ms = pct(0.99);
+        double s = 0;
+        for (double d : v) s += d;
+        o->mean_ms = s / (double)v.size();
+        o->max_ms = v.back();
+    }
+    return GRNS_OK;
+}
+
+grns_status grns_serve_destroy(grns_server *S) {
+    if (!S) return GRNS_ERR_ARG;
+    if (S->worker.joinable()) {
+        {
+            std::lock_guard<std::mutex> g(S->mu);
+            S->stop = true;
+        }
+        S->cv.notify_all();
+        S->worker.join();
+    }
+    for (auto &sl : S->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.d_tasks) (void)hipFree(sl.d_tasks);
+        if (sl.h_stage) (void)hipHostFree(sl.h_stage);
+        if (sl.d_counters) (void)hipFree(sl.d_counters);
+        if (sl.max_ws) (void)hipFree(sl.max_ws);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    }
+    delete S;
     return GRNS_OK;
 }
 

@@ -162,6 +162,41 @@ grns_status grns_batch_tune(grns_batch *b, void *stream, int reps, grns_batch_co
 grns_status grns_batch_launch(grns_batch *b, void *stream);
 grns_status grns_batch_destroy(grns_batch *b);
 
+/* ---------------------------------------------------------------- serving form (no reference
+ * counterpart): a request loop on one device.  A request is a short list of tasks (the batch task
+ * of above; op GRNS_OP_MAX adds the arg-max: a -> out0 (int[n] residues), out1 (long long index))
+ * whose operands and results are already resident in device memory.  A dispatcher thread admits
+ * requests in arrival order while their bytes fit the HBM budget and a slot (stream) is free,
+ * fuses the admitted requests of one moduli set into one persistent launch, and lets launches on
+ * different slots overlap on the device.  Latency = completion seen by the dispatcher minus
+ * submission. */
+#define GRNS_OP_MAX 9
+
+typedef struct {
+    size_t hbm_budget;   /* bytes of operands + results of requests in flight            */
+    int slots;           /* concurrent launches, one HIP stream each (default priority)  */
+    int max_batch;       /* requests fused into one launch                              */
+    int tile;            /* launch shape of the fused launches (see grns_batch_config)  */
+    int blocks_per_cu;
+} grns_serve_config;
+
+typedef struct {
+    long long submitted, completed, rejected, launches;
+    double p50_ms, p99_ms, mean_ms, max_ms;
+} grns_serve_stats;
+
+typedef struct grns_server grns_server;
+grns_status grns_serve_create(int device, const grns_serve_config *cfg, grns_server **out);
+/* Queue a request; *ticket identifies it.  GRNS_ERR_ADMISSION if it can never fit the budget. */
+grns_status grns_serve_submit(grns_server *s, grns_ctx *ctx, const grns_batch_task *tasks, int ntasks,
+                              long long *ticket);
+/* state: 0 queued, 1 running, 2 done, 3 failed; latency_ms valid when done. */
+grns_status grns_serve_poll(grns_server *s, long long ticket, int *state, double *latency_ms);
+grns_status grns_serve_wait(grns_server *s, long long ticket, double *latency_ms);
+grns_status grns_serve_drain(grns_server *s);
+grns_status grns_serve_stats_get(grns_server *s, grns_serve_stats *out);
+grns_status grns_serve_destroy(grns_server *s);
+
 /* Version string of the library. */
 const char *grns_version(void);
 
