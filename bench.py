@@ -70,6 +70,69 @@ def load_traffic(name):
         return None
 
 
+def batched_mpint_leg(steps, warmup, vectors=4096, max_len=65536, seed=4):
+    """4096 mpint vectors (32 moduli) of log-uniform length in [1, 65536], random signs, |x| < 2^480,
+    op add/sub/mul drawn per vector: one persistent launch (autotuned, hipGraph) against one
+    per-call launch per vector.  Every vector's result is compared between the two forms."""
+    import numpy as np
+    import torch
+
+    import grns_amd as G
+    from grns_amd.workload import Synth
+
+    mod = G.largest_primes_below(N_MODULI)
+    n = len(mod)
+    ctx = G.Context(mod, 0)
+    syn = Synth(mod)
+    rng = np.random.default_rng(seed)
+    lens = np.floor(np.exp(rng.uniform(0, np.log(max_len), size=vectors))).astype(np.int64).clip(1, max_len)
+    ops = rng.integers(0, 3, size=vectors)
+    total = int(lens.sum())
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]])
+
+    def operand(stream):
+        bits = torch.full((total,), 480, dtype=torch.int32, device="cuda")
+        d = syn.bits(bits, seed, stream, top_set=False)
+        s = syn.uniform_u32(total, 0, 1, seed, stream + 100)
+        ev = ctx.eval(d)  # interval records have the mp_int_t eval[2] layout
+        return d, s, ev.view(total, 2, 2)
+
+    X, Y = operand(10), operand(20)
+    Zb = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
+    Zp = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
+    names = ["mpint_add", "mpint_sub", "mpint_mul"]
+
+    def view(T, o, ln):
+        return (T[0][:, o:o + ln], T[1][o:o + ln], T[2][o:o + ln])
+
+    tasks = [G.make_task(names[op], int(ln), mx=view(X, o, ln), my=view(Y, o, ln), mz=view(Zb, o, ln))
+             for op, o, ln in zip(ops, offs, lens)]
+    batch = G.Batch(ctx, tasks)
+    cfg, tuned_ms = batch.tune(reps=2)
+    batch.set_config(use_graph=True)
+
+    def plain():
+        for op, o, ln in zip(ops, offs, lens):
+            ctx.mpint(names[op][6:], view(X, o, ln), view(Y, o, ln), view(Zp, o, ln), int(ln))
+
+    plain()
+    batch.launch()
+    torch.cuda.synchronize()
+    same = all(torch.equal(a, b) for a, b in zip(Zb, Zp))
+    if not same:
+        raise SystemExit("batched mpint results differ from the per-call results")
+    ms_b = timed(lambda: batch.launch(), steps, warmup)
+    ms_p = timed(plain, max(2, steps // 4), 1)
+    bytes_per = 3 * (4 * n + 36)
+    out = {"workload": f"{vectors} mpint vectors n={n}, log-uniform lengths [1,{max_len}], {total} elements, "
+                       "add/sub/mul per vector",
+           "elements": total, "batched_ms": round(ms_b, 4), "per_call_ms": round(ms_p, 4),
+           "speedup": round(ms_p / ms_b, 2), "elem_ops_per_s": round(total / (ms_b * 1e-3)),
+           "roofline": roofline(bytes_per * total, ms_b), "tuned": cfg, "bit_exact_vs_per_call": same}
+    batch.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +216,14 @@ def main():
         status = ctx.status()
         if status:
             log(f"[bench] device status bits {status}")
+        del xe, ye
+        torch.cuda.empty_cache()
+
+    if "batch" in only or not args.only:
+        extras["batched"] = batched_mpint_leg(args.steps, args.warmup)
+        log(f"[bench] batched mpint: {extras['batched']['batched_ms']} ms vs per-call "
+            f"{extras['batched']['per_call_ms']} ms")
+        torch.cuda.empty_cache()
 
     # ------------------------------------------------------------------ CPU baseline
     cpu = None

@@ -73,7 +73,7 @@ EXPORTS = [
     "grns_serve_drain", "grns_serve_stats_get", "grns_serve_destroy",
 ]
 DEBUG_EXPORTS = [
-    "grns_debug_eval_hot", "grns_debug_filter", "grns_debug_ceil_log2", "grns_debug_mulmod_recip_d",
+    "grns_debug_eval_hot", "grns_debug_filter", "grns_debug_cmp_lane", "grns_debug_ceil_log2", "grns_debug_mulmod_recip_d",
     "grns_debug_mulmod_recip", "grns_debug_addmod_recip", "grns_debug_bn_err", "grns_debug_fast_scalbn",
     "grns_debug_filter_tau", "grns_debug_filter_err",
 ]
@@ -116,6 +116,7 @@ def lib():
     L.grns_debug_filter.restype = D
     L.grns_debug_filter.argtypes = [P, P, P, P]
     L.grns_debug_ceil_log2.argtypes = [D]
+    L.grns_debug_cmp_lane.argtypes = [P, P, P]
     L.grns_debug_mulmod_recip_d.restype = D
     L.grns_debug_mulmod_recip_d.argtypes = [D, D, D, D]
     L.grns_debug_mulmod_recip.argtypes = [I, I, I, D]
@@ -300,6 +301,11 @@ class Context:
         out = np.zeros(1, INTERVAL_DTYPE)
         ok = lib().grns_debug_eval_hot(self._h, _np_ptr(a), _np_ptr(out), int(fast))
         return ok, out[0]
+
+    def debug_cmp_lane(self, x, y):
+        a = np.ascontiguousarray(x, dtype=np.int32)
+        b = np.ascontiguousarray(y, dtype=np.int32)
+        return lib().grns_debug_cmp_lane(self._h, _np_ptr(a), _np_ptr(b))
 
     def debug_filter(self, residues):
         a = np.ascontiguousarray(residues, dtype=np.int32)

@@ -36,7 +36,7 @@ struct DevCtx {
     int n;
     int log2M;
     int ref_factor;
-    int pad0;
+    int big_moduli;    // 1 when every m_i >= 2^22 (enables the near-tie certificate of comparisons)
     double accuracy;
     double tau;        // fraction window outside of which no evaluation needs MRC or refinement
     double err;        // bound on |approx fraction - x/M mod 1| of the filter sums
@@ -181,10 +181,10 @@ GRNS_HD double mulmod_recip_d(double dx, double dy, double dm, double d) {
     double di = trunc(p * d);
     double lo = fma(dx, dy, -p);   // x*y - p, exact
     double t = fma(-di, dm, p);    // p - i*m, exact
-    double s = t + lo;             // x*y - i*m, exact
-    s = (s >= 2147483648.0) ? s - 4294967296.0 : s;
-    s = (s < -2147483648.0) ? s + 4294967296.0 : s;
-    return s;
+    double s = t + lo;             // x*y - i*m, exact, |s| < 2^34
+    // int32 wrap: subtract 2^32 * floor(s / 2^32 + 1/2) (every step exact: s has <= 34 bits)
+    double k = floor(fma(s, 0x1p-32, 0.5));
+    return fma(-k, 4294967296.0, s);
 }
 GRNS_HD int mulmod_recip(int x, int y, int m, double d) {
     double q = ((double)x * (double)y) * d;

@@ -84,6 +84,17 @@ GRNS_D void store_interval(interval_t *p, const interval_t &v) {
 
 template <int N>
 GRNS_D void load_number(const int *__restrict__ x, long long smod, long long snum, long long j, int (&xr)[N]) {
+    if (snum == 1 && j < (1ll << 29)) {
+        // modulus-major: one wave-uniform row pointer per modulus plus a 32-bit lane offset, so
+        // each load is a scalar-base global load with no per-load address arithmetic
+        const unsigned jj = (unsigned)j;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const int *row = x + (long long)i * smod;
+            xr[i] = __builtin_nontemporal_load(row + jj);
+        }
+        return;
+    }
     const int *p = x + j * snum;
 #pragma unroll
     for (int i = 0; i < N; i++) xr[i] = __builtin_nontemporal_load(p + (long long)i * smod);
@@ -174,7 +185,7 @@ GRNS_HD double fract_d(double v) {
 #endif
 }
 
-template <int N>
+template <int N, bool REG = true>
 GRNS_HD double filter_frac(const DevCtx *c, const int (&xr)[N], bool &regular, bool &canon) {
     double acc = 0.0, dev = 0.0;
     bool cn = true;
@@ -186,16 +197,79 @@ GRNS_HD double filter_frac(const DevCtx *c, const int (&xr)[N], bool &regular, b
         double e = fma(dx, c->chi[i], -P);
         double t = fma(dx, c->clo[i], e);
         double fi = fract_d(P) + t;
-        dev = fmax(dev, fabs(fi - 0.5));
+        if (REG) dev = fmax(dev, fabs(fi - 0.5));
         acc += fi;
     }
     double F = fract_d(acc);
     canon = cn;
-    regular = cn && dev < 0.5 - 0x1p-20 && F > c->tau && F < 1.0 - c->tau;
+    regular = REG && cn && dev < 0.5 - 0x1p-20 && F > c->tau && F < 1.0 - c->tau;
     return F;
 }
 
+// Exact sign of x - y for two canonical numbers known to lie within M/4 of each other, or 2 when
+// this cannot be certified.  d = x - y mod M is formed residue-wise; with s_i = d_i w_i mod m_i
+// and K the integer nearest to sum s_i / m_i, T = sum s_i (M/m_i) - K M equals x - y over the
+// integers whenever |x - y| is small, and T is computed mod 2^64.  T is accepted only when
+// |T| < 2^62 and T = d_i mod m_i for every i: then T = x - y (mod M) and |T| < M/4, so T is x - y.
+template <int N>
+GRNS_HD int near_sign(const DevCtx *c, const int (&xr)[N], const int (&yr)[N]) {
+    if (!c->big_moduli) return 2;  // the 64-bit remainders below need m_i >= 2^22
+    unsigned long long T = 0;
+    double acc = 0.0;
+    int dd[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        int m = c->m[i];
+        int d = xr[i] - yr[i];
+        d += (d < 0) ? m : 0;
+        dd[i] = d;
+        // s = d * w mod m, canonical
+        double p = (double)d * c->dw[i];
+        double q = floor(p * c->dinv[i]);
+        double s = fma(-q, c->dm[i], p) + fma((double)d, c->dw[i], -p);
+        s += (s < 0) ? c->dm[i] : 0.0;
+        s -= (s >= c->dm[i]) ? c->dm[i] : 0.0;
+        acc = fma(s, c->dinv[i], acc);
+        T += (unsigned long long)(unsigned)(int)s * c->mi64[i];
+    }
+    double K = rint(acc);
+    T -= (unsigned long long)(long long)K * c->M64;
+    long long Ts = (long long)T;
+    if (Ts >= (1ll << 62) || Ts <= -(1ll << 62)) return 2;
+    int th = (int)(Ts >> 32);
+    unsigned tl = (unsigned)Ts;
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        long long v = (long long)th * (long long)c->c32[i] + (long long)tl;  // T mod m_i, up to a multiple of m_i
+        long long r = rem_trunc(v, c->m[i], c->dinv[i]);
+        r += (r < 0) ? c->m[i] : 0;
+        ok &= (r == dd[i]);
+    }
+    if (!ok) return 2;
+    return (Ts > 0) - (Ts < 0);
+}
+
 // ---------------------------------------------------------------- comparison
+// rns_cmp for the numbers held by this lane, or 2 when the reference algorithm itself has to run
+// (a number whose CPU-path evaluation is not regular, or a near-tie the certificate cannot settle).
+template <int N>
+GRNS_HD int cmp_lane(const DevCtx *c, const int (&xr)[N], const int (&yr)[N]) {
+    bool rx, ry, cx, cy;
+    double fx = filter_frac<N>(c, xr, rx, cx);
+    double fy = filter_frac<N>(c, yr, ry, cy);
+    if (!(rx && ry)) return 2;
+    // Both CPU-path evaluations are valid enclosures: rns_cmp returns the exact sign.
+    double e2 = 2.0 * c->err;
+    if (fx - fy > e2) return 1;
+    if (fy - fx > e2) return -1;
+    bool eq = true;
+#pragma unroll
+    for (int i = 0; i < N; i++) eq &= (xr[i] == yr[i]);
+    if (eq) return 0;
+    return near_sign<N>(c, xr, yr);
+}
+
 // One comparison per lane (number j, valid when j < count); all 64 lanes of the wave call it.
 template <int N>
 GRNS_D void cmp_step(const DevCtx *__restrict__ c, const int *__restrict__ x, long long xsm, long long xsn,
@@ -207,24 +281,8 @@ GRNS_D void cmp_step(const DevCtx *__restrict__ c, const int *__restrict__ x, lo
             int xr[N], yr[N];
             load_number<N>(x, xsm, xsn, j, xr);
             load_number<N>(y, ysm, ysn, j, yr);
-            bool rx, ry, cx, cy;
-            double fx = filter_frac<N>(c, xr, rx, cx);
-            double fy = filter_frac<N>(c, yr, ry, cy);
-            int res = 0;
-            if (rx && ry) {
-                // Both CPU-path evaluations are valid enclosures: rns_cmp returns the exact sign.
-                double e2 = 2.0 * c->err;
-                if (fx - fy > e2) res = 1;
-                else if (fy - fx > e2) res = -1;
-                else {
-                    bool eq = true;
-#pragma unroll
-                    for (int i = 0; i < N; i++) eq &= (xr[i] == yr[i]);
-                    if (!eq) rare = true;
-                }
-            } else {
-                rare = true;
-            }
+            int res = cmp_lane<N>(c, xr, yr);
+            rare = (res == 2);
             if (!rare) sign[j] = res;
         }
         unsigned long long mask = __ballot(rare);
@@ -355,7 +413,7 @@ GRNS_D void max_merge(const DevCtx *c, int G, const int *__restrict__ x, long lo
 template <int N>
 GRNS_D MaxCand max_key(const DevCtx *c, const int (&xr)[N], long long j) {
     bool reg, canon;
-    double F = filter_frac<N>(c, xr, reg, canon);
+    double F = filter_frac<N, false>(c, xr, reg, canon);
     MaxCand e;
     e.idx = j;
     double E = c->err;

@@ -115,6 +115,8 @@ grns_status build_constants(grns_ctx *c, const int *moduli, int n) {
     while (tau < 4 * h.accuracy) tau *= 2;
     h.tau = tau;
     h.err = 0x1p-30;
+    h.big_moduli = 1;
+    for (int i = 0; i < n; i++) h.big_moduli &= (moduli[i] >= (1 << 22));
     // RNS_EVAL_UNIT / INV_UNIT: M rounded down to 53 bits, then 1/Md and 1 - 1/Md rounded up and
     // down to 53 bits in the [0.5, 1) convention.
     {
@@ -485,6 +487,12 @@ int host_eval_hot_n(const grns_ctx *c, const int *x, grns_interval *out, int fas
     return ok ? 1 : 0;
 }
 template <int N>
+int host_cmp_n(const grns_ctx *c, const int *x, const int *y) {
+    int xr[N], yr[N];
+    for (int i = 0; i < N; i++) { xr[i] = x[i]; yr[i] = y[i]; }
+    return cmp_lane<N>(&c->h, xr, yr);
+}
+template <int N>
 double host_filter_n(const grns_ctx *c, const int *x, int *regular, int *canon) {
     int xr[N];
     for (int i = 0; i < N; i++) xr[i] = x[i];
@@ -517,6 +525,17 @@ double grns_debug_filter(const grns_ctx *c, const int *x, int *regular, int *can
         case 32: return host_filter_n<32>(c, x, regular, canon);
         case 64: return host_filter_n<64>(c, x, regular, canon);
         default: *regular = 0; *canon = 0; return -1.0;
+    }
+}
+/* In-lane decision of the comparison kernel: 1, 0, -1, or 2 when the cooperative path runs. */
+int grns_debug_cmp_lane(const grns_ctx *c, const int *x, const int *y) {
+    switch (c->n) {
+        case 4: return host_cmp_n<4>(c, x, y);
+        case 8: return host_cmp_n<8>(c, x, y);
+        case 16: return host_cmp_n<16>(c, x, y);
+        case 32: return host_cmp_n<32>(c, x, y);
+        case 64: return host_cmp_n<64>(c, x, y);
+        default: return -2;
     }
 }
 int grns_debug_ceil_log2(double s) { return ceil_log2_libm(s); }

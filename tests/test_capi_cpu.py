@@ -165,3 +165,31 @@ def test_scalbn_and_reductions_match_oracle():
             assert L.grns_debug_mulmod_recip(x, y, m, d) == want
             assert L.grns_debug_mulmod_recip_d(float(x), float(y), float(m), d) == float(want)
             assert L.grns_debug_addmod_recip(x, y, m, d) == O.orc_mod_add(x, y, m, d)
+
+
+@pytest.mark.parametrize("n", [8, 32, 64])
+def test_cmp_lane_decisions_exact(n):
+    """The in-lane comparison (filter, equality, near-tie certificate) is the exact sign whenever it
+    decides, agrees with the CPU restatement, and settles the +-1 near-ties itself."""
+    mod = SETS[n]
+    M = G.product(mod)
+    c = G.Context(mod, device=-1)
+    o = Oracle(mod)
+    rnd = random.Random(300 + n)
+    decided_near = 0
+    for k in range(3000):
+        x = rnd.randrange(M)
+        kind = k % 4
+        y = x + (1, -1, 0, None)[kind] if kind < 3 else rnd.randrange(M)
+        if not 0 <= y < M:
+            y = x
+        if k % 97 == 0:
+            y = (x + rnd.randrange(1 << 70)) % M
+        xr, yr = [x % m for m in mod], [y % m for m in mod]
+        r = c.debug_cmp_lane(xr, yr)
+        if r == 2:
+            continue
+        assert r == (x > y) - (x < y), (x, y)
+        assert r == o.cmp1(xr, yr)
+        decided_near += kind < 2
+    assert decided_near > 1400
