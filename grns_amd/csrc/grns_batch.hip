@@ -498,6 +498,7 @@ struct Slot {
     bool busy = false;
     std::vector<Request> reqs;
     size_t bytes = 0;
+    Clock::time_point sim_done;
 };
 
 }  // namespace
@@ -516,11 +517,16 @@ struct grns_server {
     std::thread worker;
     std::vector<double> lat;
     long long submitted = 0, completed = 0, rejected = 0, launches = 0;
+    int sim_delay_us = -1;  // >= 0: host simulation of launches (scheduler tests without a GPU)
 };
 
 namespace {
 
 grns_status launch_group(grns_server *S, Slot &sl) {
+    if (S->sim_delay_us >= 0) {  // host simulation: the launch "completes" after a fixed delay
+        sl.sim_done = Clock::now() + std::chrono::microseconds(S->sim_delay_us);
+        return GRNS_OK;
+    }
     std::vector<grns_batch_task> flat;
     grns_ctx *c = sl.reqs.front().ctx;
     for (auto &r : sl.reqs)
@@ -547,7 +553,8 @@ grns_status launch_group(grns_server *S, Slot &sl) {
 }
 
 void serve_loop(grns_server *S) {
-    (void)hipSetDevice(S->device);
+    const bool sim = S->sim_delay_us >= 0;
+    if (!sim) (void)hipSetDevice(S->device);
     std::unique_lock<std::mutex> lk(S->mu);
     for (;;) {
         bool progress = false;
@@ -555,7 +562,7 @@ void serve_loop(grns_server *S) {
         for (auto &sl : S->slots) {
             if (!sl.busy) continue;
             lk.unlock();
-            hipError_t q = hipEventQuery(sl.done);
+            hipError_t q = sim ? (Clock::now() >= sl.sim_done ? hipSuccess : hipErrorNotReady) : hipEventQuery(sl.done);
             lk.lock();
             if (q == hipErrorNotReady) continue;
             auto now = Clock::now();
@@ -609,7 +616,7 @@ void serve_loop(grns_server *S) {
                     t.result = st;
                     S->completed++;
                 }
-                (void)hipStreamSynchronize(sl.stream);
+                if (!sim) (void)hipStreamSynchronize(sl.stream);
                 S->budget_free += sl.bytes;
                 sl.busy = false;
                 sl.reqs.clear();
@@ -657,6 +664,25 @@ grns_status grns_serve_create(int device, const grns_serve_config *cfg, grns_ser
             return GRNS_ERR_OOM;
         }
     }
+    S->worker = std::thread(serve_loop, S);
+    *out = S;
+    return GRNS_OK;
+}
+
+/* Scheduler test hook: a server whose launches run nothing and complete delay_us after they are
+ * issued; it takes host-only contexts (device -1). */
+grns_status grns_serve_create_sim(const grns_serve_config *cfg, int delay_us, grns_server **out) {
+    if (!cfg || !out || cfg->slots < 1 || cfg->slots > 16 || cfg->max_batch < 1 || cfg->hbm_budget == 0 ||
+        delay_us < 0)
+        return GRNS_ERR_ARG;
+    grns_server *S = new (std::nothrow) grns_server();
+    if (!S) return GRNS_ERR_OOM;
+    S->device = -1;
+    S->cfg = *cfg;
+    S->budget_free = cfg->hbm_budget;
+    S->sim_delay_us = delay_us;
+    S->slots.resize((size_t)cfg->slots);
+    for (auto &sl : S->slots) sl.cap = 8 * cfg->max_batch;
     S->worker = std::thread(serve_loop, S);
     *out = S;
     return GRNS_OK;

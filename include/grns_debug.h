@@ -21,6 +21,9 @@ double grns_debug_bn_err(double c);
 double grns_debug_fast_scalbn(double x, int n);
 double grns_debug_filter_tau(const grns_ctx *ctx);
 double grns_debug_filter_err(const grns_ctx *ctx);
+/* Scheduler test hook: a server whose launches run nothing and complete delay_us after issue;
+ * it accepts host-only contexts (device -1), so admission and batching run without a GPU. */
+grns_status grns_serve_create_sim(const grns_serve_config *cfg, int delay_us, grns_server **out);
 #ifdef __cplusplus
 }
 #endif
