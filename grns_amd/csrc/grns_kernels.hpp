@@ -206,6 +206,29 @@ GRNS_HD double filter_frac(const DevCtx *c, const int (&xr)[N], bool &regular, b
     return F;
 }
 
+// Exact sign of x - y for two canonical numbers known to lie within M/4 of each other when their
+// difference is small: t = (x_0 - y_0) reduced into (-m_0/2, m_0/2] is x - y exactly iff
+// t = x_i - y_i (mod m_i) for every i (then t = x - y mod M, and both lie within M/4 of 0).
+// Returns 2 when the check fails (|x - y| >= m_0/2).
+template <int N>
+GRNS_HD int small_diff_sign(const DevCtx *c, const int (&xr)[N], const int (&yr)[N]) {
+    const int m0 = c->m[0];
+    int t = xr[0] - yr[0];  // (-m0, m0)
+    if (t > m0 / 2) t -= m0;
+    else if (t <= -((m0 + 1) / 2)) t += m0;
+    bool ok = true;
+#pragma unroll
+    for (int i = 1; i < N; i++) {
+        int m = c->m[i];
+        int d = xr[i] - yr[i];
+        d += (d < 0) ? m : 0;
+        int tm = t + ((t < 0) ? m : 0);  // t mod m_i, valid for |t| < m_i
+        ok &= (d == tm) && (2 * (long long)(t < 0 ? -t : t) < (long long)m);
+    }
+    if (!ok) return 2;
+    return (t > 0) - (t < 0);
+}
+
 // Exact sign of x - y for two canonical numbers known to lie within M/4 of each other, or 2 when
 // this cannot be certified.  d = x - y mod M is formed residue-wise; with s_i = d_i w_i mod m_i
 // and K the integer nearest to sum s_i / m_i, T = sum s_i (M/m_i) - K M equals x - y over the
@@ -267,7 +290,8 @@ GRNS_HD int cmp_lane(const DevCtx *c, const int (&xr)[N], const int (&yr)[N]) {
 #pragma unroll
     for (int i = 0; i < N; i++) eq &= (xr[i] == yr[i]);
     if (eq) return 0;
-    return near_sign<N>(c, xr, yr);
+    int s = small_diff_sign<N>(c, xr, yr);
+    return s != 2 ? s : near_sign<N>(c, xr, yr);
 }
 
 // One comparison per lane (number j, valid when j < count); all 64 lanes of the wave call it.
@@ -380,12 +404,43 @@ GRNS_D int max_decide(const MaxCand &e, const MaxCand &best) {
     return 2;
 }
 
-// Every lane proposes (e -> best); pending lanes are resolved by the groups of the wave.  All
-// 64 lanes of the wave must call this.
-GRNS_D void max_merge(const DevCtx *c, int G, const int *__restrict__ x, long long smod, long long snum, MaxCand &best,
+// Exact order of numbers a and b of x in one lane: 0 equal, +-1, or 2 when the lane cannot settle
+// it (then a wave group runs mixed-radix conversion).  `near` tells that both numbers are known to
+// lie within M/4 of each other (both keys finite and overlapping), which the certificates need.
+template <int N>
+GRNS_D int lane_cmp_exact(const DevCtx *c, const int *__restrict__ x, long long smod, long long snum, long long a,
+                          long long b, bool near) {
+    int xa[N], xb[N];
+    bool eq = true, canon = true;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        xa[i] = x[a * snum + (long long)i * smod];
+        xb[i] = x[b * snum + (long long)i * smod];
+        eq &= xa[i] == xb[i];
+        canon &= (unsigned)xa[i] < (unsigned)c->m[i] && (unsigned)xb[i] < (unsigned)c->m[i];
+    }
+    if (eq) return 0;
+    if (!near || !canon) return 2;
+    int s = small_diff_sign<N>(c, xa, xb);
+    return s != 2 ? s : near_sign<N>(c, xa, xb);
+}
+
+// Every lane proposes (e -> best).  Undecided lanes first try the in-lane exact order; what remains
+// is resolved by the groups of the wave.  All 64 lanes of the wave must call this.
+template <int N>
+GRNS_D void max_merge(const DevCtx *c, const int *__restrict__ x, long long smod, long long snum, MaxCand &best,
                       const MaxCand &e, bool valid) {
+    const int G = Pow2Ceil<N>::v;
     int d = valid ? max_decide(e, best) : 0;
     bool take = (d == 1);
+    if (d == 2) {
+        bool near = e.lo >= 0 && best.lo >= 0;
+        int r = lane_cmp_exact<N>(c, x, smod, snum, e.idx, best.idx, near);
+        if (r != 2) {
+            take = (r > 0) || (r == 0 && e.idx < best.idx);
+            d = 0;
+        }
+    }
     unsigned long long mask = __ballot(d == 2);
     if (mask) {
         Group g = make_group(G, c->n);
@@ -427,7 +482,8 @@ GRNS_D MaxCand max_key(const DevCtx *c, const int (&xr)[N], long long j) {
     return e;
 }
 
-GRNS_D void block_max_reduce(const DevCtx *c, int G, const int *__restrict__ x, long long smod, long long snum,
+template <int N>
+GRNS_D void block_max_reduce(const DevCtx *c, const int *__restrict__ x, long long smod, long long snum,
                              MaxCand &best, MaxCand *sh) {
     sh[threadIdx.x] = best;
     __syncthreads();
@@ -436,7 +492,7 @@ GRNS_D void block_max_reduce(const DevCtx *c, int G, const int *__restrict__ x, 
         MaxCand mine = sh[threadIdx.x];
         __syncthreads();
         MaxCand b = mine;
-        max_merge(c, G, x, smod, snum, b, other, threadIdx.x < (unsigned)h);
+        max_merge<N>(c, x, smod, snum, b, other, threadIdx.x < (unsigned)h);
         if (threadIdx.x < (unsigned)h) sh[threadIdx.x] = b;
         __syncthreads();
     }
@@ -449,7 +505,6 @@ __global__ __launch_bounds__(BLOCK) void k_max_partial(const DevCtx *__restrict_
                                                        long long smod, long long snum, long long count,
                                                        long long chunk, MaxCand *__restrict__ part) {
     __shared__ MaxCand sh[BLOCK];
-    const int G = Pow2Ceil<N>::v;
     long long start = (long long)blockIdx.x * chunk;
     long long end = start + chunk < count ? start + chunk : count;
     MaxCand best;
@@ -464,14 +519,15 @@ __global__ __launch_bounds__(BLOCK) void k_max_partial(const DevCtx *__restrict_
             load_number<N>(x, smod, snum, j, xr);
             e = max_key<N>(c, xr, j);
         }
-        max_merge(c, G, x, smod, snum, best, e, valid);
+        max_merge<N>(c, x, smod, snum, best, e, valid);
     }
-    block_max_reduce(c, G, x, smod, snum, best, sh);
+    block_max_reduce<N>(c, x, smod, snum, best, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = best;
 }
 
 // Stage 2: one block reduces the per-block candidates and writes the winner.
-static __global__ __launch_bounds__(BLOCK) void k_max_final(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_max_final(const DevCtx *__restrict__ c, const int *__restrict__ x,
                                                      long long smod, long long snum, const MaxCand *__restrict__ part,
                                                      int nparts, int *__restrict__ out, long long *__restrict__ out_idx) {
     __shared__ MaxCand sh[BLOCK];
@@ -483,9 +539,9 @@ static __global__ __launch_bounds__(BLOCK) void k_max_final(const DevCtx *__rest
         e.idx = -1; e.lo = -1.0; e.hi = -1.0;
         bool valid = k < nparts;
         if (valid) e = part[k];
-        max_merge(c, G, x, smod, snum, best, e, valid && e.idx >= 0);
+        max_merge<N>(c, x, smod, snum, best, e, valid && e.idx >= 0);
     }
-    block_max_reduce(c, G, x, smod, snum, best, sh);
+    block_max_reduce<N>(c, x, smod, snum, best, sh);
     if (threadIdx.x < (unsigned)c->n && best.idx >= 0) out[threadIdx.x] = x[best.idx * snum + (long long)threadIdx.x * smod];
     if (threadIdx.x == 0) *out_idx = best.idx;
 }

@@ -405,10 +405,12 @@ grns_status grns_max(grns_ctx *c, const int *x, long long smod, long long snum, 
     MaxCand *part = reinterpret_cast<MaxCand *>(workspace);
     dim3 grid((unsigned)nb), blk(BLOCK);
     long long cnt = (long long)count;
-    int G = groups_for(c->n);
 #define GRNS_MAX_CASE(NN)                                                                                    \
     case NN:                                                                                                 \
         hipLaunchKernelGGL((k_max_partial<NN>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, ch, part); \
+        if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                            \
+        hipLaunchKernelGGL((k_max_final<NN>), dim3(1), blk, 0, S(stream), c->d, x, smod, snum, part, (int)nb, \
+                           out_residues, out_idx);                                                           \
         break;
     switch (c->n) {
         GRNS_MAX_CASE(4)
@@ -420,9 +422,6 @@ grns_status grns_max(grns_ctx *c, const int *x, long long smod, long long snum, 
             return GRNS_ERR_UNSUPPORTED;
     }
 #undef GRNS_MAX_CASE
-    if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;
-    hipLaunchKernelGGL(k_max_final, dim3(1), blk, 0, S(stream), c->d, G, x, smod, snum, part, (int)nb, out_residues,
-                       out_idx);
     return launch_check();
 }
 
