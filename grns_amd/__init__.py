@@ -73,7 +73,8 @@ EXPORTS = [
     "grns_serve_drain", "grns_serve_stats_get", "grns_serve_destroy",
 ]
 DEBUG_EXPORTS = [
-    "grns_debug_eval_hot", "grns_debug_filter", "grns_debug_cmp_lane", "grns_debug_ceil_log2", "grns_debug_mulmod_recip_d",
+    "grns_debug_eval_hot", "grns_debug_filter", "grns_debug_cmp_lane", "grns_debug_max_key", "grns_debug_ceil_log2",
+    "grns_debug_mulmod_recip_d",
     "grns_debug_mulmod_recip", "grns_debug_addmod_recip", "grns_debug_bn_err", "grns_debug_fast_scalbn",
     "grns_debug_filter_tau", "grns_debug_filter_err",
 ]
@@ -125,6 +126,8 @@ def lib():
     L.grns_debug_bn_err.argtypes = [D]
     L.grns_debug_fast_scalbn.restype = D
     L.grns_debug_fast_scalbn.argtypes = [D, I]
+    L.grns_debug_max_key.restype = D
+    L.grns_debug_max_key.argtypes = [P, P]
     L.grns_debug_filter_tau.restype = D
     L.grns_debug_filter_tau.argtypes = [P]
     L.grns_debug_filter_err.restype = D
@@ -306,6 +309,10 @@ class Context:
         a = np.ascontiguousarray(x, dtype=np.int32)
         b = np.ascontiguousarray(y, dtype=np.int32)
         return lib().grns_debug_cmp_lane(self._h, _np_ptr(a), _np_ptr(b))
+
+    def debug_max_key(self, residues):
+        a = np.ascontiguousarray(residues, dtype=np.int32)
+        return lib().grns_debug_max_key(self._h, _np_ptr(a))
 
     def debug_filter(self, residues):
         a = np.ascontiguousarray(residues, dtype=np.int32)

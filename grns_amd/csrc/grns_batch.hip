@@ -55,8 +55,11 @@ GRNS_D bool next_tile(const BatchTask *__restrict__ tasks, int ntasks, long long
         s_k = v < total ? find_task(tasks, ntasks, v) : 0;
     }
     __syncthreads();
-    t = s_t;
-    k = s_k;
+    // the same for the whole block: keep them in scalar registers (task fields become scalar loads)
+    const unsigned long long tv = (unsigned long long)s_t;
+    t = (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(tv >> 32)) << 32) |
+                    (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)tv));
+    k = __builtin_amdgcn_readfirstlane(s_k);
     __syncthreads();
     return t < total;
 }

@@ -16,6 +16,70 @@
 #define GRNS_D __device__ __forceinline__
 
 namespace grns {
+// Hides the value of a wave-uniform 64-bit integer from the optimiser at this point: uses after it
+// cannot be hoisted above it, and the value is kept in scalar registers.  The value must be the
+// same on every lane of the wave (lane 0's is taken, which also moves a value the compiler keeps
+// in vector registers into scalar ones).
+GRNS_HD long long uniform_point(long long v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((unsigned long long)v >> 32));
+    v = (long long)(((unsigned long long)hi << 32) | lo);
+    asm volatile("" : "+s"(v));
+#endif
+    return v;
+}
+
+// The same for a wave-uniform pointer, so that constant table loads placed after it are issued
+// there (scalar loads from the scalar cache) instead of being hoisted out of the streaming loops,
+// where dozens of per-modulus constants would exceed the scalar register file and spill into
+// vector registers.
+template <class T>
+GRNS_HD const T *reload_point(const T *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return reinterpret_cast<const T *>(uniform_point(reinterpret_cast<long long>(p)));
+#else
+    return p;
+#endif
+}
+
+// A per-lane 32-bit value the optimiser may not widen or fold: a byte offset that stays a 32-bit
+// vector offset of scalar-base global loads.
+GRNS_HD unsigned lane_offset(unsigned v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(v));
+#endif
+    return v;
+}
+
+// A view of a constant table (wave-uniform address) whose loads are scalar loads issued where the
+// view is taken: the address goes through uniform_point, and the constant address space lets the
+// compiler use the scalar cache.  On the host it is the plain pointer.  Pass an address computed
+// from reload_point(ctx) so that not even the address is hoisted out of the caller's loop.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(4))) double cdouble;
+typedef const __attribute__((address_space(4))) int cint;
+#else
+typedef const double cdouble;
+typedef const int cint;
+#endif
+GRNS_HD cdouble *const_view(const double *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return reinterpret_cast<cdouble *>(uniform_point(reinterpret_cast<long long>(p)));
+#else
+    return p;
+#endif
+}
+GRNS_HD cint *const_view(const int *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return reinterpret_cast<cint *>(uniform_point(reinterpret_cast<long long>(p)));
+#else
+    return p;
+#endif
+}
+}  // namespace grns
+
+namespace grns {
 
 // er_float_t / interval_t of src/types.cuh:13-24 (long is 64-bit on the LP64 targets).
 struct er_t {
@@ -50,6 +114,8 @@ struct DevCtx {
     double ru[MAXN];             // RNS_MODULI_RECIP_RU
     double chi[MAXN];            // w_i / m_i as a double-double (filter sums)
     double clo[MAXN];
+    double khi[MAXN];            // w_i / m_i truncated to 22 significant bits: x_i * khi is exact
+    double klo[MAXN];            // w_i / m_i - khi (arg-max keys)
     int c32[MAXN];               // 2^32 mod m_i
     unsigned long long mi64[MAXN];  // (M / m_i) mod 2^64
     unsigned long long M64;         // M mod 2^64
@@ -366,19 +432,34 @@ GRNS_D long long group_sum_ll(const Group &g, long long v) {
 }
 
 // Mixed-radix digits (src/mrc.cuh:31-44), pipelined: at step j the final digit j is broadcast
-// and every higher lane applies the same update the serial loop applies to it.
+// and every higher lane applies the same update the serial loop applies to it.  The inverses of
+// eight steps are requested together, so that the table's memory latency is paid once per eight
+// steps rather than once per step.
 GRNS_D int group_mrc(const DevCtx *c, const Group &g, int xi) {
     int mr = xi;
     const int n = c->n;
     const bool act = g.gl < g.L;
     const int mi = act ? c->m[g.gl] : 1;
     const double dinv = act ? c->dinv[g.gl] : 1.0;
-    for (int j = 0; j < g.L - 1; j++) {
-        int mrj = bcast_i(mr, g.base + j);
-        if (act && g.gl > j) {
-            if (mr < mrj) mr = (int)((long long)mi - (long long)mrj + (long long)mr);
-            else mr = (int)((unsigned)mr - (unsigned)mrj);
-            mr = mulmod3(mr, c->mrc_inv[j * n + g.gl], mi, dinv);
+    const int *tab = c->mrc_inv;
+    const int steps = g.L - 1;
+    for (int j0 = 0; j0 < steps; j0 += 8) {
+        int inv[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int j = j0 + k;
+            inv[k] = (act && j < steps && g.gl > j) ? tab[j * n + g.gl] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int j = j0 + k;
+            if (j >= steps) break;  // the same on every lane of the group
+            int mrj = bcast_i(mr, g.base + j);
+            if (act && g.gl > j) {
+                if (mr < mrj) mr = (int)((long long)mi - (long long)mrj + (long long)mr);
+                else mr = (int)((unsigned)mr - (unsigned)mrj);
+                mr = mulmod3(mr, inv[k], mi, dinv);
+            }
         }
     }
     return mr;
@@ -411,6 +492,7 @@ GRNS_D void group_refine(const DevCtx *c, const Group &g, int &s, double sumu, i
     const double dinv = act ? c->dinv[g.gl] : 1.0;
     long long K = 0;
     int it = 0;
+    int kprev = -1, p2 = 0;
     while (sumu < c->accuracy) {
         // MAX(-(ceil(log2(sumu))+1), REF): log2 of a negative sum is NaN and selects REF; a zero
         // sum gives +inf, which the reference turns into an index outside RNS_POW2.
@@ -426,7 +508,10 @@ GRNS_D void group_refine(const DevCtx *c, const Group &g, int &s, double sumu, i
             return;
         }
         int k = (int)kv;
-        int p2 = act ? c->pow2[(size_t)k * c->n + g.gl] : 0;
+        if (k != kprev) {  // the scale factor repeats from step to step: its row is read once
+            p2 = act ? c->pow2[(size_t)k * c->n + g.gl] : 0;
+            kprev = k;
+        }
         s = mulmod_recip(s, p2, mi, dinv);
         double fu = ddiv_ru((double)s, dm);
         sumu = group_psum<true>(g, fu);
@@ -445,9 +530,11 @@ GRNS_D void group_refine(const DevCtx *c, const Group &g, int &s, double sumu, i
 // rns_eval_compute (FAST=false, src/rnseval.cuh:29-102) and rns_eval_compute_fast (FAST=true,
 // :112-167) for the number whose residue of modulus gl is xi.  The result is the same on all
 // lanes of the group.
+// mrd/has_mrd receive this lane's mixed-radix digit when the evaluation computed the digits.
 template <bool FAST>
-GRNS_D interval_t group_eval(const DevCtx *c, const Group &g, int xi, int &status) {
+GRNS_D interval_t group_eval(const DevCtx *c, const Group &g, int xi, int &status, int &mrd, bool &has_mrd) {
     interval_t out;
+    has_mrd = false;
     const bool act = g.gl < g.L;
     if (group_all(g, !act || xi == 0)) {
         out.low.frac = 0; out.low.exp = 0; out.upp.frac = 0; out.upp.exp = 0;
@@ -477,6 +564,8 @@ GRNS_D interval_t group_eval(const DevCtx *c, const Group &g, int xi, int &statu
     int mr = -1;
     if (whl != whu) {
         int d = group_mrc(c, g, xi);
+        mrd = d;
+        has_mrd = true;
         mr = bcast_i(d, g.base + g.L - 1);
     }
     if (mr > 0) { out.upp = c->inv_upp; return out; }
@@ -486,15 +575,25 @@ GRNS_D interval_t group_eval(const DevCtx *c, const Group &g, int xi, int &statu
     return out;
 }
 
-// rns_cmp, src/rnscmp.cuh:20-39
+template <bool FAST>
+GRNS_D interval_t group_eval(const DevCtx *c, const Group &g, int xi, int &status) {
+    int d;
+    bool h;
+    return group_eval<FAST>(c, g, xi, status, d, h);
+}
+
+// rns_cmp, src/rnscmp.cuh:20-39 (the digits an evaluation computed are reused by the final
+// mixed-radix comparison: both come from the same conversion of the same residues)
 GRNS_D int group_cmp(const DevCtx *c, const Group &g, int xi, int yi, int &status) {
-    interval_t ex = group_eval<false>(c, g, xi, status);
-    interval_t ey = group_eval<false>(c, g, yi, status);
+    int dx = 0, dy = 0;
+    bool hx, hy;
+    interval_t ex = group_eval<false>(c, g, xi, status, dx, hx);
+    interval_t ey = group_eval<false>(c, g, yi, status, dy, hy);
     if (er_ucmp(ex.low, ey.upp) > 0) return 1;
     if (er_ucmp(ey.low, ex.upp) > 0) return -1;
     if (group_all(g, g.gl >= g.L || xi == yi)) return 0;
-    int mx = group_mrc(c, g, xi);
-    int my = group_mrc(c, g, yi);
+    int mx = hx ? dx : group_mrc(c, g, xi);
+    int my = hy ? dy : group_mrc(c, g, yi);
     return group_mrs_cmp(g, mx, my);
 }
 

@@ -1,6 +1,7 @@
 // grns_internal.hpp -- library-internal declarations shared by the host translation units.
 #pragma once
 
+#include <mutex>
 #include <vector>
 
 #include "../../include/grns.h"
@@ -15,6 +16,15 @@ struct grns_ctx {
     int *d_pow2 = nullptr;
     int *d_mrc = nullptr;
     int *d_status = nullptr;
+    int max_blocks = 2048;         // arg-max stage-1 grid: one resident wave of blocks on the device
+    // rare-item queues of the per-call eval / cmp kernels, one per stream (made on first use)
+    struct RareQueue {
+        void *stream;
+        long long *items;
+        unsigned *ctr;
+    };
+    std::mutex qmu;
+    std::vector<RareQueue> queues;
     std::vector<int> pow2;
     std::vector<int> mrc;
 };

@@ -46,11 +46,22 @@ template <int N>
 struct EvalLeaf {
     const DevCtx *c;
     const int *xr;
+    // the trees ask for leaves in ascending order: the constants come in views of 4 moduli
+    cdouble *dw = nullptr, *dm = nullptr, *dinv = nullptr, *rd = nullptr, *ru = nullptr;
     template <int I>
     GRNS_HD void get(double &lo, double &up) {
-        double s = mulmod_recip_d((double)xr[I], c->dw[I], c->dm[I], c->dinv[I]);
-        lo = dmul_rd(s, c->rd[I]);
-        up = dmul_ru(s, c->ru[I]);
+        if constexpr (I % 4 == 0) {
+            const DevCtx *cc = reload_point(c);
+            dw = const_view(cc->dw + I);
+            dm = const_view(cc->dm + I);
+            dinv = const_view(cc->dinv + I);
+            rd = const_view(cc->rd + I);
+            ru = const_view(cc->ru + I);
+        }
+        constexpr int k = I % 4;
+        double s = mulmod_recip_d((double)xr[I], dw[k], dm[k], dinv[k]);
+        lo = dmul_rd(s, rd[k]);
+        up = dmul_ru(s, ru[k]);
     }
 };
 
@@ -82,16 +93,23 @@ GRNS_D void store_interval(interval_t *p, const interval_t &v) {
     q[1] = make_double2(v.upp.frac, __longlong_as_double(v.upp.exp));
 }
 
+// A pointer into global memory (address space 1): loads through it are global loads, which take a
+// scalar base and a 32-bit vector offset.
+typedef const __attribute__((address_space(1))) int global_int;
+
 template <int N>
 GRNS_D void load_number(const int *__restrict__ x, long long smod, long long snum, long long j, int (&xr)[N]) {
     if (snum == 1 && j < (1ll << 29)) {
-        // modulus-major: one wave-uniform row pointer per modulus plus a 32-bit lane offset, so
-        // each load is a scalar-base global load with no per-load address arithmetic
-        const unsigned jj = (unsigned)j;
+        // modulus-major: a wave-uniform row pointer stepped from modulus to modulus plus one 32-bit
+        // byte offset, so each load is a scalar-base global load; the row pointer is recomputed
+        // per modulus (reload_point) rather than N of them being held in scalar registers
+        const unsigned off = lane_offset((unsigned)j * 4u);
+        const long long step = smod * 4;
+        long long row = reinterpret_cast<long long>(x);
 #pragma unroll
         for (int i = 0; i < N; i++) {
-            const int *row = x + (long long)i * smod;
-            xr[i] = __builtin_nontemporal_load(row + jj);
+            xr[i] = __builtin_nontemporal_load(reinterpret_cast<const global_int *>(row + off));
+            row = uniform_point(row + step);
         }
         return;
     }
@@ -100,9 +118,62 @@ GRNS_D void load_number(const int *__restrict__ x, long long smod, long long snu
     for (int i = 0; i < N; i++) xr[i] = __builtin_nontemporal_load(p + (long long)i * smod);
 }
 
-// Resolve the lanes of `mask` with whole-group evaluations.
+// ---------------------------------------------------------------- rare-item queue
+// The per-call kernels leave the numbers (or pairs) whose result needs the full reference
+// algorithm to a second kernel: the streaming pass appends their indices here, and the rare pass
+// spreads them over every wave group of the device, so that a few long evaluations (refinement
+// loops, mixed-radix conversions) neither stall the streaming waves nor run one after another in
+// the wave that met them.  ctr[0] counts pushes, ctr[1] counts finished rare-pass blocks (the last
+// one clears the counters for the next call on the stream), ctr[2] is set when the queue was full;
+// then the overflowing items carry a marker in their output and the rare pass scans for it.
+struct RareQ {
+    long long *items;
+    unsigned *ctr;
+    unsigned cap;
+};
+
+// Appends j for the lanes with `rare` (all 64 lanes of the wave call it, one atomic per wave).
+// Returns false for a lane whose item did not fit (the caller then writes the marker).
+GRNS_D bool rare_push(const RareQ &q, bool rare, long long j) {
+    const unsigned long long m = __ballot(rare);
+    if (!m) return true;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(&q.ctr[0], (unsigned)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (!rare) return true;
+    const unsigned slot = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+    if (slot < q.cap) {
+        q.items[slot] = j;
+        return true;
+    }
+    q.ctr[2] = 1u;
+    return false;
+}
+
+// End of a rare pass: the last block to finish clears the counters.
+GRNS_D void rare_done(const RareQ &q) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned d = atomicAdd(&q.ctr[1], 1u);
+        if (d == gridDim.x - 1) {
+            q.ctr[0] = 0u;
+            q.ctr[2] = 0u;
+            q.ctr[1] = 0u;
+            __threadfence();
+        }
+    }
+}
+
+// Marker of an evaluation left to the rare pass after a queue overflow (no result has it).
+constexpr long long EVAL_MARK = 0x7ff0dead0000beefll;
+
+// Resolve the lanes of `mask` with whole-group evaluations.  Not inlined: the rare path's
+// registers then do not add to those of the streaming loop that calls it.
 template <bool FAST>
-GRNS_D void coop_eval_lanes(const DevCtx *c, int G, int L, const int *__restrict__ x, long long smod, long long snum,
+__device__ __attribute__((noinline)) void coop_eval_lanes(const DevCtx *c, int G, int L, const int *__restrict__ x, long long smod, long long snum,
                             long long j, unsigned long long mask, interval_t *__restrict__ out, int *status) {
     Group g = make_group(G, L);
     const int P = 64 / G;
@@ -139,12 +210,61 @@ GRNS_D void eval_step(const DevCtx *__restrict__ c, const int *__restrict__ x, l
     if (mask) coop_eval_lanes<FAST>(c, Pow2Ceil<N>::v, N, x, smod, snum, j, mask, out, status);
 }
 
+// Plain per-call evaluation, streaming pass: one number per lane; numbers that need the
+// cooperative algorithm go to the rare-item queue.
 template <int N, bool FAST>
 __global__ __launch_bounds__(BLOCK) void k_eval(const DevCtx *__restrict__ c, const int *__restrict__ x, long long smod,
                                                 long long snum, long long count, interval_t *__restrict__ out,
-                                                int *__restrict__ status) {
-    for (long long base = (long long)blockIdx.x * BLOCK; base < count; base += (long long)gridDim.x * BLOCK)
-        eval_step<N, FAST>(c, x, smod, snum, base + threadIdx.x, count, out, status);
+                                                RareQ q) {
+    for (long long base = (long long)blockIdx.x * BLOCK; base < count; base += (long long)gridDim.x * BLOCK) {
+        const long long j = base + threadIdx.x;
+        bool rare = false;
+        if (j < count) {
+            int xr[N];
+            load_number<N>(x, smod, snum, j, xr);
+            interval_t r;
+            if (eval_hot<N, FAST>(c, xr, r)) store_interval(out + j, r);
+            else rare = true;
+        }
+        if (!rare_push(q, rare, j)) {
+            interval_t mk;
+            mk.low.frac = __builtin_nan(""); mk.low.exp = EVAL_MARK;
+            mk.upp.frac = __builtin_nan(""); mk.upp.exp = EVAL_MARK;
+            store_interval(out + j, mk);
+        }
+    }
+}
+
+// Rare pass of the evaluation: one wave group per queued number (then, after an overflow, per
+// marked output).
+template <bool FAST>
+__global__ __launch_bounds__(BLOCK) void k_eval_rare(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
+                                                     long long smod, long long snum, long long count,
+                                                     interval_t *__restrict__ out, int *__restrict__ status, RareQ q) {
+    Group g = make_group(G, c->n);
+    const int P = 64 / G;
+    const long long gid = (((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6) * P + (threadIdx.x & 63) / G;
+    const long long ngroups = ((long long)gridDim.x * BLOCK >> 6) * P;
+    const unsigned pushed = q.ctr[0];
+    const long long nq = pushed < q.cap ? pushed : q.cap;
+    int st = 0;
+    for (long long k = gid; k < nq; k += ngroups) {
+        const long long j = q.items[k];
+        int xi = g.gl < g.L ? x[j * snum + (long long)g.gl * smod] : 0;
+        interval_t r = group_eval<FAST>(c, g, xi, st);
+        if (g.gl == 0) store_interval(out + j, r);
+    }
+    if (q.ctr[2]) {  // the queue overflowed: the remaining numbers carry the marker
+        for (long long j = gid; j < count; j += ngroups) {
+            const long long e = reinterpret_cast<const long long *>(out + j)[1];
+            if (e != EVAL_MARK) continue;
+            int xi = g.gl < g.L ? x[j * snum + (long long)g.gl * smod] : 0;
+            interval_t r = group_eval<FAST>(c, g, xi, st);
+            if (g.gl == 0) store_interval(out + j, r);
+        }
+    }
+    if (g.gl == 0 && st) atomicOr(status, st);
+    rare_done(q);
 }
 
 // Any n <= 64: one group per number throughout.
@@ -190,15 +310,23 @@ GRNS_HD double filter_frac(const DevCtx *c, const int (&xr)[N], bool &regular, b
     double acc = 0.0, dev = 0.0;
     bool cn = true;
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-        double dx = (double)xr[i];
-        cn &= (unsigned)xr[i] < (unsigned)c->m[i];
-        double P = dx * c->chi[i];
-        double e = fma(dx, c->chi[i], -P);
-        double t = fma(dx, c->clo[i], e);
-        double fi = fract_d(P) + t;
-        if (REG) dev = fmax(dev, fabs(fi - 0.5));
-        acc += fi;
+    for (int i0 = 0; i0 < N; i0 += 8) {
+        const DevCtx *cc = reload_point(c);  // 8 moduli of constants at a time
+        cdouble *ch = const_view(cc->chi + i0);
+        cdouble *cl = const_view(cc->clo + i0);
+        cint *cm = const_view(cc->m + i0);
+#pragma unroll
+        for (int k = 0; k < 8 && i0 + k < N; k++) {
+            const int xi = xr[i0 + k];
+            double dx = (double)xi;
+            cn &= (unsigned)xi < (unsigned)cm[k];
+            double P = dx * ch[k];
+            double e = fma(dx, ch[k], -P);
+            double t = fma(dx, cl[k], e);
+            double fi = fract_d(P) + t;
+            if (REG) dev = fmax(dev, fabs(fi - 0.5));
+            acc += fi;
+        }
     }
     double F = fract_d(acc);
     canon = cn;
@@ -234,18 +362,19 @@ GRNS_HD int small_diff_sign(const DevCtx *c, const int (&xr)[N], const int (&yr)
 // and K the integer nearest to sum s_i / m_i, T = sum s_i (M/m_i) - K M equals x - y over the
 // integers whenever |x - y| is small, and T is computed mod 2^64.  T is accepted only when
 // |T| < 2^62 and T = d_i mod m_i for every i: then T = x - y (mod M) and |T| < M/4, so T is x - y.
-template <int N>
-GRNS_HD int near_sign(const DevCtx *c, const int (&xr)[N], const int (&yr)[N]) {
+// The residues are read from memory (px[i * sx], py[i * sy]) in two passes: this is a rare
+// path, and keeping it out of registers leaves the callers' hot loops their occupancy.
+template <int U = 4>
+GRNS_HD int near_sign(const DevCtx *c, const int *px, long long sx, const int *py, long long sy) {
     if (!c->big_moduli) return 2;  // the 64-bit remainders below need m_i >= 2^22
+    const int n = c->n;
     unsigned long long T = 0;
     double acc = 0.0;
-    int dd[N];
-#pragma unroll
-    for (int i = 0; i < N; i++) {
+#pragma unroll U
+    for (int i = 0; i < n; i++) {
         int m = c->m[i];
-        int d = xr[i] - yr[i];
+        int d = px[i * sx] - py[i * sy];
         d += (d < 0) ? m : 0;
-        dd[i] = d;
         // s = d * w mod m, canonical
         double p = (double)d * c->dw[i];
         double q = floor(p * c->dinv[i]);
@@ -262,12 +391,15 @@ GRNS_HD int near_sign(const DevCtx *c, const int (&xr)[N], const int (&yr)[N]) {
     int th = (int)(Ts >> 32);
     unsigned tl = (unsigned)Ts;
     bool ok = true;
-#pragma unroll
-    for (int i = 0; i < N; i++) {
+#pragma unroll U
+    for (int i = 0; i < n; i++) {
+        int m = c->m[i];
+        int d = px[i * sx] - py[i * sy];
+        d += (d < 0) ? m : 0;
         long long v = (long long)th * (long long)c->c32[i] + (long long)tl;  // T mod m_i, up to a multiple of m_i
-        long long r = rem_trunc(v, c->m[i], c->dinv[i]);
-        r += (r < 0) ? c->m[i] : 0;
-        ok &= (r == dd[i]);
+        long long r = rem_trunc(v, m, c->dinv[i]);
+        r += (r < 0) ? m : 0;
+        ok &= (r == d);
     }
     if (!ok) return 2;
     return (Ts > 0) - (Ts < 0);
@@ -275,13 +407,33 @@ GRNS_HD int near_sign(const DevCtx *c, const int (&xr)[N], const int (&yr)[N]) {
 
 // ---------------------------------------------------------------- comparison
 // rns_cmp for the numbers held by this lane, or 2 when the reference algorithm itself has to run
-// (a number whose CPU-path evaluation is not regular, or a near-tie the certificate cannot settle).
-template <int N>
-GRNS_HD int cmp_lane(const DevCtx *c, const int (&xr)[N], const int (&yr)[N]) {
+// (a number whose CPU-path evaluation is not regular, or a near-tie the certificates cannot
+// settle).  xr/yr hold the residues; px/py (sx/sy apart) are where they were read from.  NEAR
+// false leaves out the 64-bit certificate (its pairs then return 2).
+template <int N, bool NEAR = true>
+GRNS_HD int cmp_lane(const DevCtx *c, const int (&xr)[N], const int (&yr)[N], const int *px, long long sx,
+                     const int *py, long long sy) {
     bool rx, ry, cx, cy;
     double fx = filter_frac<N>(c, xr, rx, cx);
     double fy = filter_frac<N>(c, yr, ry, cy);
-    if (!(rx && ry)) return 2;
+    if (!(rx && ry)) {
+        // Outside the regular case the reference's evaluations have to be run, except in two cases
+        // whose rns_cmp result does not depend on them: identical residues (the same interval twice,
+        // low <= upp, so neither bound test holds and the residue test returns 0), and a zero
+        // against a canonical nonzero number (zero evaluates to [0, 0] exactly, the other's upper
+        // bound is >= 0, and either its lower bound is > 0 or the exact mixed-radix order decides).
+        bool eq = true, zx = true, zy = true;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            eq &= (xr[i] == yr[i]);
+            zx &= (xr[i] == 0);
+            zy &= (yr[i] == 0);
+        }
+        if (eq) return 0;
+        if (zx && cy) return -1;
+        if (zy && cx) return 1;
+        return 2;
+    }
     // Both CPU-path evaluations are valid enclosures: rns_cmp returns the exact sign.
     double e2 = 2.0 * c->err;
     if (fx - fy > e2) return 1;
@@ -291,7 +443,35 @@ GRNS_HD int cmp_lane(const DevCtx *c, const int (&xr)[N], const int (&yr)[N]) {
     for (int i = 0; i < N; i++) eq &= (xr[i] == yr[i]);
     if (eq) return 0;
     int s = small_diff_sign<N>(c, xr, yr);
-    return s != 2 ? s : near_sign<N>(c, xr, yr);
+    if (!NEAR) return s;
+    return s != 2 ? s : near_sign<2>(c, px, sx, py, sy);
+}
+
+// The lanes of `mask` run the reference comparison with whole groups.
+template <int N>
+GRNS_D void coop_cmp_lanes(const DevCtx *c, const int *__restrict__ x, long long xsm,
+                                                         long long xsn, const int *__restrict__ y, long long ysm,
+                                                         long long ysn, long long j, unsigned long long mask,
+                                                         int *__restrict__ sign, int *__restrict__ status) {
+    const int G = Pow2Ceil<N>::v;
+    Group g = make_group(G, N);
+    const int P = 64 / G;
+    const int gid = (threadIdx.x & 63) / G;
+    while (mask) {
+        int src = kth_bit(mask, gid);
+        long long jl = __shfl(j, src < 0 ? 0 : src, 64);
+        mask = drop_low_bits(mask, P);
+        if (src >= 0) {
+            int xi = g.gl < N ? x[jl * xsn + (long long)g.gl * xsm] : 0;
+            int yi = g.gl < N ? y[jl * ysn + (long long)g.gl * ysm] : 0;
+            int st = 0;
+            int r = group_cmp(c, g, xi, yi, st);
+            if (g.gl == 0) {
+                sign[jl] = r;
+                if (st) atomicOr(status, st);
+            }
+        }
+    }
 }
 
 // One comparison per lane (number j, valid when j < count); all 64 lanes of the wave call it.
@@ -305,43 +485,70 @@ GRNS_D void cmp_step(const DevCtx *__restrict__ c, const int *__restrict__ x, lo
             int xr[N], yr[N];
             load_number<N>(x, xsm, xsn, j, xr);
             load_number<N>(y, ysm, ysn, j, yr);
-            int res = cmp_lane<N>(c, xr, yr);
+            int res = cmp_lane<N>(c, xr, yr, x + j * xsn, xsm, y + j * ysn, ysm);
             rare = (res == 2);
             if (!rare) sign[j] = res;
         }
         unsigned long long mask = __ballot(rare);
-        if (mask) {
-            const int G = Pow2Ceil<N>::v;
-            Group g = make_group(G, N);
-            const int P = 64 / G;
-            const int gid = (threadIdx.x & 63) / G;
-            while (mask) {
-                int src = kth_bit(mask, gid);
-                long long jl = __shfl(j, src < 0 ? 0 : src, 64);
-                mask = drop_low_bits(mask, P);
-                if (src >= 0) {
-                    int xi = g.gl < N ? x[jl * xsn + (long long)g.gl * xsm] : 0;
-                    int yi = g.gl < N ? y[jl * ysn + (long long)g.gl * ysm] : 0;
-                    int st = 0;
-                    int r = group_cmp(c, g, xi, yi, st);
-                    if (g.gl == 0) {
-                        sign[jl] = r;
-                        if (st) atomicOr(status, st);
-                    }
-                }
-            }
-        }
+        if (mask) coop_cmp_lanes<N>(c, x, xsm, xsn, y, ysm, ysn, j, mask, sign, status);
     }
 }
 
+// Plain per-call comparison, streaming pass: one pair per lane, no cooperative code (so the loop
+// keeps its registers); pairs the lane cannot settle go to the rare-item queue.
 template <int N>
 __global__ __launch_bounds__(BLOCK) void k_cmp(const DevCtx *__restrict__ c, const int *__restrict__ x, long long xsm,
                                                long long xsn, const int *__restrict__ y, long long ysm, long long ysn,
-                                               long long count, int *__restrict__ sign, int *__restrict__ status) {
-    for (long long base = (long long)blockIdx.x * BLOCK; base < count; base += (long long)gridDim.x * BLOCK)
-        cmp_step<N>(c, x, xsm, xsn, y, ysm, ysn, base + threadIdx.x, count, sign, status);
+                                               long long count, int *__restrict__ sign, RareQ q) {
+    for (long long base = (long long)blockIdx.x * BLOCK; base < count; base += (long long)gridDim.x * BLOCK) {
+        const long long j = base + threadIdx.x;
+        int res = 0;
+        if (j < count) {
+            int xr[N], yr[N];
+            load_number<N>(x, xsm, xsn, j, xr);
+            load_number<N>(y, ysm, ysn, j, yr);
+            res = cmp_lane<N, false>(c, xr, yr, x + j * xsn, xsm, y + j * ysn, ysm);
+            if (res != 2) sign[j] = res;
+        }
+        if (!rare_push(q, res == 2, j)) sign[j] = 2;  // marker: left to the scan of the rare pass
+    }
 }
 
+// Rare pass of the comparison: per queued pair (then, after an overflow, per marked output) one
+// wave group runs the reference algorithm.
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_cmp_rare(const DevCtx *__restrict__ c, const int *__restrict__ x,
+                                                    long long xsm, long long xsn, const int *__restrict__ y,
+                                                    long long ysm, long long ysn, long long count,
+                                                    int *__restrict__ sign, int *__restrict__ status, RareQ q) {
+    const int G = Pow2Ceil<N>::v;
+    Group g = make_group(G, N);
+    const int P = 64 / G;
+    const long long gid = (((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6) * P + (threadIdx.x & 63) / G;
+    const long long ngroups = ((long long)gridDim.x * BLOCK >> 6) * P;
+    const unsigned pushed = q.ctr[0];
+    const long long nq = pushed < q.cap ? pushed : q.cap;
+    const bool scan = q.ctr[2] != 0;
+    const long long total = nq + (scan ? count : 0);
+    int st = 0;
+    for (long long k = gid; k < total; k += ngroups) {
+        long long j;
+        if (k < nq) {
+            j = q.items[k];
+        } else {
+            j = k - nq;
+            if (sign[j] != 2) continue;
+        }
+        const int xi = g.gl < N ? x[j * xsn + (long long)g.gl * xsm] : 0;
+        const int yi = g.gl < N ? y[j * ysn + (long long)g.gl * ysm] : 0;
+        const int r = group_cmp(c, g, xi, yi, st);
+        if (g.gl == 0) sign[j] = r;
+    }
+    if (g.gl == 0 && st) atomicOr(status, st);
+    rare_done(q);
+}
+
+// Any n <= 64: one group per pair throughout.
 static __global__ __launch_bounds__(BLOCK) void k_cmp_group(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
                                                      long long xsm, long long xsn, const int *__restrict__ y,
                                                      long long ysm, long long ysn, long long count,
@@ -404,75 +611,123 @@ GRNS_D int max_decide(const MaxCand &e, const MaxCand &best) {
     return 2;
 }
 
-// Exact order of numbers a and b of x in one lane: 0 equal, +-1, or 2 when the lane cannot settle
-// it (then a wave group runs mixed-radix conversion).  `near` tells that both numbers are known to
-// lie within M/4 of each other (both keys finite and overlapping), which the certificates need.
-template <int N>
-GRNS_D int lane_cmp_exact(const DevCtx *c, const int *__restrict__ x, long long smod, long long snum, long long a,
-                          long long b, bool near) {
-    int xa[N], xb[N];
-    bool eq = true, canon = true;
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-        xa[i] = x[a * snum + (long long)i * smod];
-        xb[i] = x[b * snum + (long long)i * smod];
-        eq &= xa[i] == xb[i];
-        canon &= (unsigned)xa[i] < (unsigned)c->m[i] && (unsigned)xb[i] < (unsigned)c->m[i];
+// Exact order (sign of x_a - x_b) settled within one lane: 0 for equal residues, +-1 from the
+// difference certificates when `near` (both numbers known to lie within M/4 of each other), else
+// 2, and a wave group then runs mixed-radix conversion.  pa/pb point at residue 0 of the two
+// numbers, smod apart.  The residues are read from memory: this path is rare and stays out of the
+// registers of the streaming loops that call it.
+template <int U>
+GRNS_D int lane_cmp_exact(const DevCtx *c, const int *pa, const int *pb, long long smod, bool near) {
+    const int n = c->n;
+    const int m0 = c->m[0];
+    const int a0 = pa[0], b0 = pb[0];
+    bool eq = a0 == b0;
+    bool canon = (unsigned)a0 < (unsigned)m0 && (unsigned)b0 < (unsigned)m0;
+    int t = (int)((unsigned)a0 - (unsigned)b0);
+    if (t > m0 / 2) t -= m0;
+    else if (t <= -((m0 + 1) / 2)) t += m0;
+    const long long at = t < 0 ? -(long long)t : (long long)t;
+    bool small = true;
+#pragma unroll U
+    for (int i = 1; i < n; i++) {
+        const int m = c->m[i];
+        const int ai = pa[i * smod], bi = pb[i * smod];
+        eq &= ai == bi;
+        canon &= (unsigned)ai < (unsigned)m && (unsigned)bi < (unsigned)m;
+        int d = (int)((unsigned)ai - (unsigned)bi);
+        d += (d < 0) ? m : 0;
+        const int tm = t + ((t < 0) ? m : 0);
+        small &= (d == tm) && (2 * at < (long long)m);
     }
     if (eq) return 0;
     if (!near || !canon) return 2;
-    int s = small_diff_sign<N>(c, xa, xb);
-    return s != 2 ? s : near_sign<N>(c, xa, xb);
+    if (small) return (t > 0) - (t < 0);
+    return near_sign<U>(c, pa, smod, pb, smod);
 }
 
-// Every lane proposes (e -> best).  Undecided lanes first try the in-lane exact order; what remains
-// is resolved by the groups of the wave.  All 64 lanes of the wave must call this.
+// Lanes with r == 2 hand their pair (a, b) to the groups of the wave, which order it by mixed-radix
+// digits; r receives the answer.  All 64 lanes of the wave must call coop_resolve; the group work
+// is not inlined, so that its registers do not add to those of the streaming loops.
+template <int N>
+__device__ __attribute__((noinline)) int coop_resolve_lanes(const DevCtx *c, const int *__restrict__ x, long long smod,
+                                                            long long snum, long long a, long long b, int r,
+                                                            unsigned long long mask) {
+    const int G = Pow2Ceil<N>::v;
+    Group g = make_group(G, c->n);
+    const int P = 64 / G;
+    const int gid = (threadIdx.x & 63) / G;
+    const int lane = threadIdx.x & 63;
+    while (mask) {
+        int src = kth_bit(mask, gid);  // the same on every lane of a group
+        long long ea = __shfl(a, src < 0 ? 0 : src, 64);
+        long long eb = __shfl(b, src < 0 ? 0 : src, 64);
+        mask = drop_low_bits(mask, P);
+        int rr = 0;
+        if (src >= 0) rr = group_cmp_exact(c, g, x, smod, snum, ea, eb);
+        // hand each group's answer back to the lane that asked
+        for (int k = 0; k < P; k++) {
+            int s = __shfl(src, k * G, 64);
+            int ans = __shfl(rr, k * G, 64);
+            if (s >= 0 && lane == s) r = ans;
+        }
+    }
+    return r;
+}
+
+template <int N>
+GRNS_D void coop_resolve(const DevCtx *c, const int *__restrict__ x, long long smod, long long snum, long long a,
+                         long long b, int &r) {
+    unsigned long long mask = __ballot(r == 2);
+    if (mask) r = coop_resolve_lanes<N>(c, x, smod, snum, a, b, r, mask);
+}
+
+constexpr int LANE_UNROLL = 1;  // the stage-1 loop keeps its occupancy: overlaps there are rare
+
+// Every lane proposes (e -> best): keys decide, then the exact order (lowest index among equal
+// values).  All 64 lanes of the wave must call this.
 template <int N>
 GRNS_D void max_merge(const DevCtx *c, const int *__restrict__ x, long long smod, long long snum, MaxCand &best,
                       const MaxCand &e, bool valid) {
-    const int G = Pow2Ceil<N>::v;
     int d = valid ? max_decide(e, best) : 0;
-    bool take = (d == 1);
-    if (d == 2) {
-        bool near = e.lo >= 0 && best.lo >= 0;
-        int r = lane_cmp_exact<N>(c, x, smod, snum, e.idx, best.idx, near);
-        if (r != 2) {
-            take = (r > 0) || (r == 0 && e.idx < best.idx);
-            d = 0;
-        }
-    }
-    unsigned long long mask = __ballot(d == 2);
-    if (mask) {
-        Group g = make_group(G, c->n);
-        const int P = 64 / G;
-        const int gid = (threadIdx.x & 63) / G;
-        const int lane = threadIdx.x & 63;
-        while (mask) {
-            int src = kth_bit(mask, gid);  // the same on every lane of a group
-            long long ea = __shfl(e.idx, src < 0 ? 0 : src, 64);
-            long long bb = __shfl(best.idx, src < 0 ? 0 : src, 64);
-            mask = drop_low_bits(mask, P);
-            int r = 0;
-            if (src >= 0) r = group_cmp_exact(c, g, x, smod, snum, ea, bb);
-            // hand each group's answer back to the lane that asked
-            for (int k = 0; k < P; k++) {
-                int s = __shfl(src, k * G, 64);
-                int ans = __shfl(r, k * G, 64);
-                if (s >= 0 && lane == s) take = (ans > 0) || (ans == 0 && e.idx < best.idx);
-            }
-        }
-    }
+    int r = 0;
+    if (d == 2)
+        r = lane_cmp_exact<LANE_UNROLL>(c, x + e.idx * snum, x + best.idx * snum, smod, e.lo >= 0 && best.lo >= 0);
+    coop_resolve<N>(c, x, smod, snum, e.idx, best.idx, r);
+    bool take = (d == 1) || (d == 2 && (r > 0 || (r == 0 && e.idx < best.idx)));
     if (take) best = e;
+}
+
+// Approximation F of x/M mod 1 for the arg-max keys: sum_i frac(x_i khi_i) + sum_i x_i klo_i, where
+// khi_i (22 significant bits) makes x_i khi_i exact and |x_i klo_i| < 2^9.  For canonical x,
+// |F - x/M mod 1| <= n 2^-38 (the fma chain, |B| <= n 2^9) + n 2^-43 (klo) + 2^-37 (the last
+// sums) < 2^-31 up to n = 64.  Five VALU operations per residue.
+// The sum depends on the residues only mod m_i (w_i/m_i times a multiple of m_i is an integer), so
+// the key holds for non-canonical int32 residues too, which then stand for their canonical value.
+template <int N>
+GRNS_HD double key_frac(const DevCtx *c, const int (&xr)[N]) {
+    double A = 0.0, B = 0.0;
+#pragma unroll
+    for (int i0 = 0; i0 < N; i0 += 8) {
+        const DevCtx *cc = reload_point(c);  // 8 moduli of constants at a time
+        cdouble *kh = const_view(cc->khi + i0);
+        cdouble *kl = const_view(cc->klo + i0);
+#pragma unroll
+        for (int k = 0; k < 8 && i0 + k < N; k++) {
+            double dx = (double)xr[i0 + k];
+            A += fract_d(dx * kh[k]);
+            B = fma(dx, kl[k], B);
+        }
+    }
+    return fract_d(A + B);
 }
 
 template <int N>
 GRNS_D MaxCand max_key(const DevCtx *c, const int (&xr)[N], long long j) {
-    bool reg, canon;
-    double F = filter_frac<N, false>(c, xr, reg, canon);
+    double F = key_frac<N>(c, xr);
     MaxCand e;
     e.idx = j;
     double E = c->err;
-    if (canon && F > 2.0 * E && F < 1.0 - 2.0 * E) {
+    if (F > 2.0 * E && F < 1.0 - 2.0 * E) {
         e.lo = F - E;
         e.hi = F + E;
     } else {
@@ -525,25 +780,82 @@ __global__ __launch_bounds__(BLOCK) void k_max_partial(const DevCtx *__restrict_
     if (threadIdx.x == 0) part[blockIdx.x] = best;
 }
 
-// Stage 2: one block reduces the per-block candidates and writes the winner.
+// Pivot order for stage 2: the larger lower key bound, then the lower index.
+GRNS_D bool pivot_before(const MaxCand &a, const MaxCand &b) {
+    if (a.idx < 0) return false;
+    if (b.idx < 0) return true;
+    return a.lo > b.lo || (a.lo == b.lo && a.idx < b.idx);
+}
+
+constexpr int MAX_FINAL_PER_THREAD = 32;  // stage 2 takes at most BLOCK * 32 candidates
+
+// Stage 2 (one block): the exact maximum of the per-block candidates, by rounds.  The live
+// candidate with the largest lower key bound is the pivot; every other live candidate is ordered
+// against it (keys when they separate, else exactly: in-lane certificates, then wave groups).
+// Candidates below or equal to the pivot leave (equal ones lower the tie index); those above
+// stay live for the next round.  When none is above, the pivot's value is the maximum and the
+// tie index is the lowest index holding it.
 template <int N>
 __global__ __launch_bounds__(BLOCK) void k_max_final(const DevCtx *__restrict__ c, const int *__restrict__ x,
                                                      long long smod, long long snum, const MaxCand *__restrict__ part,
                                                      int nparts, int *__restrict__ out, long long *__restrict__ out_idx) {
     __shared__ MaxCand sh[BLOCK];
-    MaxCand best;
-    best.lo = -1.0; best.hi = -1.0; best.idx = -1;
-    for (int b0 = 0; b0 < nparts; b0 += BLOCK) {
-        int k = b0 + threadIdx.x;
-        MaxCand e;
-        e.idx = -1; e.lo = -1.0; e.hi = -1.0;
-        bool valid = k < nparts;
-        if (valid) e = part[k];
-        max_merge<N>(c, x, smod, snum, best, e, valid && e.idx >= 0);
+    __shared__ unsigned long long s_tie;
+    const int T = threadIdx.x;
+    const int per = (nparts + BLOCK - 1) / BLOCK;
+    unsigned live = 0;
+    for (int t = 0; t < per; t++) {
+        int k = T + t * BLOCK;
+        if (k < nparts && part[k].idx >= 0) live |= 1u << t;
     }
-    block_max_reduce<N>(c, x, smod, snum, best, sh);
-    if (threadIdx.x < (unsigned)c->n && best.idx >= 0) out[threadIdx.x] = x[best.idx * snum + (long long)threadIdx.x * smod];
-    if (threadIdx.x == 0) *out_idx = best.idx;
+    long long win = -1;
+    for (;;) {
+        MaxCand b;
+        b.idx = -1; b.lo = -2.0; b.hi = -2.0;
+        for (int t = 0; t < per; t++)
+            if (live >> t & 1) {
+                MaxCand e = part[T + t * BLOCK];
+                if (pivot_before(e, b)) b = e;
+            }
+        sh[T] = b;
+        if (T == 0) s_tie = ~0ull;
+        __syncthreads();
+        for (int h = BLOCK / 2; h >= 1; h >>= 1) {
+            if (T < h && pivot_before(sh[T + h], sh[T])) sh[T] = sh[T + h];
+            __syncthreads();
+        }
+        const MaxCand P = sh[0];
+        if (P.idx < 0) break;  // uniform: every candidate has left
+        bool above = false;
+        unsigned long long tie = (unsigned long long)P.idx;
+        for (int t = 0; t < per; t++) {  // uniform trip count: the wave groups may be needed
+            const bool mine = live >> t & 1;
+            MaxCand e;
+            e.idx = -1; e.lo = -1.0; e.hi = -1.0;
+            if (mine) e = part[T + t * BLOCK];
+            int r = -1;
+            if (mine && e.idx != P.idx && e.hi >= P.lo)
+                r = lane_cmp_exact<N>(c, x + e.idx * snum, x + P.idx * snum, smod, e.lo >= 0 && P.lo >= 0);
+            coop_resolve<N>(c, x, smod, snum, e.idx, P.idx, r);
+            if (mine) {
+                if (r > 0) {
+                    above = true;
+                } else {
+                    live &= ~(1u << t);
+                    if (r == 0 && e.idx != P.idx && (unsigned long long)e.idx < tie) tie = (unsigned long long)e.idx;
+                }
+            }
+        }
+        atomicMin(&s_tie, tie);
+        const bool any_above = __syncthreads_or(above);
+        if (!any_above) {
+            win = (long long)s_tie;
+            break;
+        }
+        __syncthreads();  // s_tie and sh are rewritten by the next round
+    }
+    if (T < (unsigned)c->n && win >= 0) out[T] = x[win * snum + (long long)T * smod];
+    if (T == 0) *out_idx = win;
 }
 
 // ---------------------------------------------------------------- division (group per number)

@@ -89,6 +89,11 @@ grns_status build_constants(grns_ctx *c, const int *moduli, int n) {
         h.chi[i] = (double)h.w[i] / (double)moduli[i];
         double rem = std::fma(-h.chi[i], (double)moduli[i], (double)h.w[i]);  // exact remainder
         h.clo[i] = rem / (double)moduli[i];
+        unsigned long long kb;
+        std::memcpy(&kb, &h.chi[i], 8);
+        kb &= ~((1ull << 31) - 1);  // keep 22 significant bits (chi is a normal double in (0, 1))
+        std::memcpy(&h.khi[i], &kb, 8);
+        h.klo[i] = (h.chi[i] - h.khi[i]) + h.clo[i];  // the difference is exact
         h.c32[i] = (int)((1ull << 32) % (unsigned long long)m);
     }
     for (int t = 0; t < THRESH; t++) {
@@ -193,6 +198,62 @@ struct DeviceGuard {
     }
 };
 
+constexpr unsigned RARE_CAP = 1u << 20;  // queued rare items per stream (8 MiB of indices)
+
+// The rare-item queue of `stream` (allocated and cleared on the first call on that stream).
+grns_status rare_queue(grns_ctx *c, void *stream, RareQ &q) {
+    std::lock_guard<std::mutex> lock(c->qmu);
+    for (auto &e : c->queues)
+        if (e.stream == stream) {
+            q.items = e.items;
+            q.ctr = e.ctr;
+            q.cap = RARE_CAP;
+            return GRNS_OK;
+        }
+    grns_ctx::RareQueue e{stream, nullptr, nullptr};
+    if (hipMalloc(&e.items, sizeof(long long) * RARE_CAP) != hipSuccess) return GRNS_ERR_OOM;
+    if (hipMalloc(&e.ctr, 4 * sizeof(unsigned)) != hipSuccess) {
+        (void)hipFree(e.items);
+        return GRNS_ERR_OOM;
+    }
+    if (hipMemset(e.ctr, 0, 4 * sizeof(unsigned)) != hipSuccess) {
+        (void)hipFree(e.items);
+        (void)hipFree(e.ctr);
+        return GRNS_ERR_HIP;
+    }
+    c->queues.push_back(e);
+    q.items = e.items;
+    q.ctr = e.ctr;
+    q.cap = RARE_CAP;
+    return GRNS_OK;
+}
+
+// Grid of a rare pass: enough wave groups for about one number in eight, at most 2048 blocks.
+long long rare_grid(size_t count) {
+    long long b = (long long)((count + 63) / 64);
+    if (b < 1) b = 1;
+    if (b > 2048) b = 2048;
+    return b;
+}
+
+// Blocks of the arg-max stage-1 kernel resident at once on the context's device.
+int max_grid_for(const grns_ctx *c) {
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)
+        return 2048;
+    hipError_t e = hipErrorInvalidValue;
+    switch (c->n) {
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<4>, BLOCK, 0); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<8>, BLOCK, 0); break;
+        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<16>, BLOCK, 0); break;
+        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<32>, BLOCK, 0); break;
+        case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<64>, BLOCK, 0); break;
+        default: break;
+    }
+    if (e != hipSuccess || per < 1) return 2048;
+    return cus * per;
+}
+
 }  // namespace
 
 extern "C" {
@@ -230,6 +291,7 @@ grns_status grns_ctx_create(const int *moduli, int n, int device, grns_ctx **out
         grns_ctx_destroy(c);
         return GRNS_ERR_HIP;
     }
+    c->max_blocks = max_grid_for(c);
     *out = c;
     return GRNS_OK;
 }
@@ -242,6 +304,10 @@ grns_status grns_ctx_destroy(grns_ctx *c) {
         if (c->d_pow2) (void)hipFree(c->d_pow2);
         if (c->d_mrc) (void)hipFree(c->d_mrc);
         if (c->d_status) (void)hipFree(c->d_status);
+        for (auto &q : c->queues) {
+            (void)hipFree(q.items);
+            (void)hipFree(q.ctr);
+        }
     }
     delete c;
     return GRNS_OK;
@@ -296,11 +362,22 @@ grns_status grns_eval(grns_ctx *c, const int *x, long long smod, long long snum,
     interval_t *o = reinterpret_cast<interval_t *>(out);
     dim3 grid((unsigned)grid_elems(count)), blk(BLOCK);
     long long cnt = (long long)count;
+    RareQ q;
 #define GRNS_EVAL_CASE(NN)                                                                                       \
-    case NN:                                                                                                     \
-        if (fast) hipLaunchKernelGGL((k_eval<NN, true>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, o, c->d_status); \
-        else hipLaunchKernelGGL((k_eval<NN, false>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, o, c->d_status);     \
-        return launch_check();
+    case NN: {                                                                                                   \
+        grns_status qs = rare_queue(c, stream, q);                                                               \
+        if (qs != GRNS_OK) return qs;                                                                            \
+        const int G = groups_for(c->n);                                                                          \
+        dim3 rg((unsigned)rare_grid(count));                                                                     \
+        if (fast) hipLaunchKernelGGL((k_eval<NN, true>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, o, q); \
+        else hipLaunchKernelGGL((k_eval<NN, false>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, o, q);     \
+        if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                                \
+        if (fast) hipLaunchKernelGGL((k_eval_rare<true>), rg, blk, 0, S(stream), c->d, G, x, smod, snum, cnt, o,   \
+                                     c->d_status, q);                                                            \
+        else hipLaunchKernelGGL((k_eval_rare<false>), rg, blk, 0, S(stream), c->d, G, x, smod, snum, cnt, o,       \
+                                c->d_status, q);                                                                 \
+        return launch_check();                                                                                   \
+    }
     switch (c->n) {
         GRNS_EVAL_CASE(4)
         GRNS_EVAL_CASE(8)
@@ -327,10 +404,18 @@ grns_status grns_cmp(grns_ctx *c, const int *x, long long xsm, long long xsn, co
     DeviceGuard guard(c->device);
     dim3 grid((unsigned)grid_elems(count)), blk(BLOCK);
     long long cnt = (long long)count;
+    RareQ q;
 #define GRNS_CMP_CASE(NN)                                                                                          \
-    case NN:                                                                                                       \
-        hipLaunchKernelGGL((k_cmp<NN>), grid, blk, 0, S(stream), c->d, x, xsm, xsn, y, ysm, ysn, cnt, sign, c->d_status); \
-        return launch_check();
+    case NN: {                                                                                                     \
+        grns_status qs = rare_queue(c, stream, q);                                                                 \
+        if (qs != GRNS_OK) return qs;                                                                              \
+        dim3 rg((unsigned)rare_grid(count));                                                                       \
+        hipLaunchKernelGGL((k_cmp<NN>), grid, blk, 0, S(stream), c->d, x, xsm, xsn, y, ysm, ysn, cnt, sign, q);        \
+        if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                                  \
+        hipLaunchKernelGGL((k_cmp_rare<NN>), rg, blk, 0, S(stream), c->d, x, xsm, xsn, y, ysm, ysn, cnt, sign,       \
+                           c->d_status, q);                                                                        \
+        return launch_check();                                                                                     \
+    }
     switch (c->n) {
         GRNS_CMP_CASE(4)
         GRNS_CMP_CASE(8)
@@ -377,8 +462,12 @@ grns_status grns_scale2pow(grns_ctx *c, const int *x, long long smod, long long 
     return launch_check();
 }
 
-static void max_shape(size_t count, long long &nblocks, long long &chunk) {
-    const long long target = 2048;
+// Stage 1 of the arg-max runs one resident wave of blocks (CUs x blocks per CU, queried at context
+// creation), each over a contiguous chunk that is a multiple of the block size.
+static void max_shape(const grns_ctx *c, size_t count, long long &nblocks, long long &chunk) {
+    long long target = c->max_blocks;
+    if (target > (long long)BLOCK * MAX_FINAL_PER_THREAD) target = (long long)BLOCK * MAX_FINAL_PER_THREAD;
+    if (target < 1) target = 1;
     long long per = (long long)((count + target - 1) / target);
     per = ((per + BLOCK - 1) / BLOCK) * BLOCK;
     if (per < BLOCK) per = BLOCK;
@@ -388,9 +477,9 @@ static void max_shape(size_t count, long long &nblocks, long long &chunk) {
 }
 
 size_t grns_max_workspace_size(const grns_ctx *c, size_t count) {
-    (void)c;
+    if (!c) return 0;
     long long nb, ch;
-    max_shape(count, nb, ch);
+    max_shape(c, count, nb, ch);
     return (size_t)nb * sizeof(MaxCand);
 }
 
@@ -399,7 +488,7 @@ grns_status grns_max(grns_ctx *c, const int *x, long long smod, long long snum, 
     if (c && c->device < 0) return GRNS_ERR_ARG;
     if (!c || !x || !out_residues || !out_idx || count == 0) return GRNS_ERR_ARG;
     long long nb, ch;
-    max_shape(count, nb, ch);
+    max_shape(c, count, nb, ch);
     if (!workspace || ws_bytes < (size_t)nb * sizeof(MaxCand)) return GRNS_ERR_ARG;
     DeviceGuard guard(c->device);
     MaxCand *part = reinterpret_cast<MaxCand *>(workspace);
@@ -489,7 +578,7 @@ template <int N>
 int host_cmp_n(const grns_ctx *c, const int *x, const int *y) {
     int xr[N], yr[N];
     for (int i = 0; i < N; i++) { xr[i] = x[i]; yr[i] = y[i]; }
-    return cmp_lane<N>(&c->h, xr, yr);
+    return cmp_lane<N>(&c->h, xr, yr, x, 1, y, 1);
 }
 template <int N>
 double host_filter_n(const grns_ctx *c, const int *x, int *regular, int *canon) {
@@ -500,6 +589,12 @@ double host_filter_n(const grns_ctx *c, const int *x, int *regular, int *canon) 
     *regular = rg;
     *canon = cn;
     return F;
+}
+template <int N>
+double host_key_n(const grns_ctx *c, const int *x) {
+    int xr[N];
+    for (int i = 0; i < N; i++) xr[i] = x[i];
+    return key_frac<N>(&c->h, xr);
 }
 }  // namespace
 
@@ -524,6 +619,17 @@ double grns_debug_filter(const grns_ctx *c, const int *x, int *regular, int *can
         case 32: return host_filter_n<32>(c, x, regular, canon);
         case 64: return host_filter_n<64>(c, x, regular, canon);
         default: *regular = 0; *canon = 0; return -1.0;
+    }
+}
+/* Arg-max key: approximation of x/M mod 1 (within the filter error bound), -1 if n is unsupported. */
+double grns_debug_max_key(const grns_ctx *c, const int *x) {
+    switch (c->n) {
+        case 4: return host_key_n<4>(c, x);
+        case 8: return host_key_n<8>(c, x);
+        case 16: return host_key_n<16>(c, x);
+        case 32: return host_key_n<32>(c, x);
+        case 64: return host_key_n<64>(c, x);
+        default: return -1.0;
     }
 }
 /* In-lane decision of the comparison kernel: 1, 0, -1, or 2 when the cooperative path runs. */

@@ -109,6 +109,31 @@ def test_filter_bounds(n):
             assert lo <= ex <= up
 
 
+@pytest.mark.parametrize("n", [4, 8, 32, 64])
+def test_max_key_bound(n):
+    """Arg-max keys: |F - x/M mod 1| < 2^-31 (inside err) for canonical residues and for any int32
+    residues, which stand for their value mod m_i; checked with exact rationals."""
+    from fractions import Fraction
+    mod = SETS[n]
+    M = G.product(mod)
+    c = G.Context(mod, device=-1)
+    err = Fraction(G.lib().grns_debug_filter_err(c.handle))
+    rnd = random.Random(400 + n)
+    vals = eval_inputs(mod, 600, seed=500 + n) + [0, 1, M - 1, M // 2, (M + 1) // 2]
+    worst = Fraction(0)
+    for k, v in enumerate(vals):
+        r = [v % m for m in mod]
+        if k % 3 == 1:  # non-canonical spellings of the same value: r_i + t m_i within int32
+            r = [ri + rnd.choice([-1, 0, 1]) * m if -2 ** 31 <= ri + m * 1 < 2 ** 31 and ri - m >= -2 ** 31 else ri
+                 for ri, m in zip(r, mod)]
+        F = c.debug_max_key(r)
+        assert 0.0 <= F <= 1.0
+        d = abs(Fraction(F) - Fraction(v % M, M))
+        d = min(d, 1 - d)
+        worst = max(worst, d)
+    assert worst < Fraction(1, 2 ** 31) < err, float(worst)
+
+
 def _rn_unfused(c):
     return PHI1 * abs(c) + ETA  # Python floats: IEEE binary64, round to nearest, no contraction
 
@@ -193,3 +218,17 @@ def test_cmp_lane_decisions_exact(n):
         assert r == o.cmp1(xr, yr)
         decided_near += kind < 2
     assert decided_near > 1400
+    # outside the regular case: identical residues and zero against a canonical nonzero number are
+    # settled in-lane (their rns_cmp result does not depend on the evaluations), the rest is not
+    edge = [0, 1, 2, M - 1, M - 2, 1 << 40, M - (1 << 40)]
+    for a in edge:
+        for b in edge:
+            xr, yr = [a % m for m in mod], [b % m for m in mod]
+            r = c.debug_cmp_lane(xr, yr)
+            if a == b or 0 in (a, b):
+                assert r == (a > b) - (a < b) == o.cmp1(xr, yr), (a, b)
+            elif r != 2:
+                assert r == o.cmp1(xr, yr), (a, b)
+    # a non-canonical zero is not the zero of the reference's zero test
+    nz = list(mod)
+    assert c.debug_cmp_lane([0] * n, nz) == 2

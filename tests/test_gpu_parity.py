@@ -171,3 +171,51 @@ def test_mpint_parity(ctxs, dev, n, op):
     assert (Z[1].cpu().numpy() == rs).all()
     assert Z[2].cpu().numpy().tobytes() == re.tobytes()
     assert (of.cpu().numpy() == ovf).all()
+
+
+def _column(mod, v, count, dev):
+    from grns_amd.workload import residues_of
+    r = torch.tensor(np.array(residues_of(v, mod), dtype=np.int64).astype(np.int32), device=dev)
+    return r.view(-1, 1).expand(-1, count).contiguous()
+
+
+@pytest.mark.parametrize("n", [8])
+def test_rare_queue_overflow(ctxs, dev, n):
+    """More rare items than the per-stream queue holds (2^20): the overflow is marked in the
+    outputs and resolved by the rare pass's scan; the next calls on the stream start clean."""
+    c, o, mod = ctxs[n]
+    from grns_amd.moduli import product
+    M = product(mod)
+    count = (1 << 20) + 4099
+    x = _column(mod, M - 1, count, dev)  # x = M - 1 needs the mixed-radix path in every number
+    y = _column(mod, M - 2, count, dev)
+    ev = c.eval(x)
+    want = o.eval_array(to_res([M - 1], mod)).tobytes()
+    g = ev.cpu().numpy().view(np.uint8).reshape(count, 32)
+    assert (g == np.frombuffer(want, np.uint8)).all()
+    sg = c.cmp(x, y).cpu().numpy()
+    assert (sg == o.cmp1(to_res([M - 1], mod)[:, 0], to_res([M - 2], mod)[:, 0])).all()
+    # the counters were cleared: an ordinary call right after is exact
+    xs, ys = cmp_pairs(mod, 3000, seed=61)
+    xa, ya = to_res(xs, mod), to_res(ys, mod)
+    assert (c.cmp(_dev(xa, dev), _dev(ya, dev)).cpu().numpy() == o.cmp_array(xa, ya)).all()
+    assert _intervals_bytes(c.eval(_dev(xa, dev))) == o.eval_array(xa).tobytes()
+    assert c.status() == 0
+
+
+def test_rare_queue_per_stream(ctxs, dev):
+    """Calls on two streams use their own rare-item queues."""
+    c, o, mod = ctxs[32]
+    xs, ys = cmp_pairs(mod, 5000, seed=62)
+    xa, ya = to_res(xs, mod), to_res(ys, mod)
+    xd, yd = _dev(xa, dev), _dev(ya, dev)
+    ref = o.cmp_array(xa, ya)
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    outs = []
+    for s in (s1, s2):
+        with torch.cuda.stream(s):
+            outs.append(c.cmp(xd, yd))
+    torch.cuda.synchronize()
+    for t in outs:
+        assert (t.cpu().numpy() == ref).all()
