@@ -74,6 +74,7 @@ EXPORTS = [
 ]
 DEBUG_EXPORTS = [
     "grns_debug_eval_hot", "grns_debug_filter", "grns_debug_cmp_lane", "grns_debug_max_key", "grns_debug_ceil_log2",
+    "grns_debug_eval_full", "grns_debug_cmp_full",
     "grns_debug_mulmod_recip_d",
     "grns_debug_mulmod_recip", "grns_debug_addmod_recip", "grns_debug_bn_err", "grns_debug_fast_scalbn",
     "grns_debug_filter_tau", "grns_debug_filter_err",
@@ -126,6 +127,10 @@ def lib():
     L.grns_debug_bn_err.argtypes = [D]
     L.grns_debug_fast_scalbn.restype = D
     L.grns_debug_fast_scalbn.argtypes = [D, I]
+    L.grns_debug_eval_full.restype = C.c_int
+    L.grns_debug_eval_full.argtypes = [P, P, P, C.c_int, P]
+    L.grns_debug_cmp_full.restype = C.c_int
+    L.grns_debug_cmp_full.argtypes = [P, P, P]
     L.grns_debug_max_key.restype = D
     L.grns_debug_max_key.argtypes = [P, P]
     L.grns_debug_filter_tau.restype = D
@@ -309,6 +314,18 @@ class Context:
         a = np.ascontiguousarray(x, dtype=np.int32)
         b = np.ascontiguousarray(y, dtype=np.int32)
         return lib().grns_debug_cmp_lane(self._h, _np_ptr(a), _np_ptr(b))
+
+    def debug_eval_full(self, residues, fast=False):
+        a = np.ascontiguousarray(residues, dtype=np.int32)
+        out = np.zeros(1, INTERVAL_DTYPE)
+        st = C.c_int()
+        ok = lib().grns_debug_eval_full(self._h, _np_ptr(a), _np_ptr(out), int(fast), C.byref(st))
+        return ok, out[0], st.value
+
+    def debug_cmp_full(self, x, y):
+        a = np.ascontiguousarray(x, dtype=np.int32)
+        b = np.ascontiguousarray(y, dtype=np.int32)
+        return lib().grns_debug_cmp_full(self._h, _np_ptr(a), _np_ptr(b))
 
     def debug_max_key(self, residues):
         a = np.ascontiguousarray(residues, dtype=np.int32)

@@ -656,7 +656,8 @@ grns_status grns_serve_create(int device, const grns_serve_config *cfg, grns_ser
     S->slots.resize((size_t)cfg->slots);
     for (auto &sl : S->slots) {
         sl.cap = 8 * cfg->max_batch;
-        sl.max_ws_bytes = 64 * 1024;
+        // the largest arg-max workspace of any context (64 moduli, the most stage-1 blocks)
+        sl.max_ws_bytes = (size_t)FBLOCK * MAX_FINAL_PER_THREAD * (sizeof(MaxCand) + 64 * sizeof(int));
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
             hipMalloc(&sl.d_tasks, sizeof(BatchTask) * sl.cap) != hipSuccess ||

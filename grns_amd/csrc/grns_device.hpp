@@ -104,6 +104,7 @@ struct DevCtx {
     double accuracy;
     double tau;        // fraction window outside of which no evaluation needs MRC or refinement
     double err;        // bound on |approx fraction - x/M mod 1| of the filter sums
+    double mr_lim;     // 1 / m_{n-1} rounded up: x/M above it makes the top mixed-radix digit positive
     er_t unit_low, unit_upp, inv_low, inv_upp;
     int m[MAXN];
     int w[MAXN];                 // RNS_PART_MODULI_PRODUCT_INVERSE
@@ -149,6 +150,15 @@ GRNS_HD double ddiv_rd(double a, double b) { double c = a / b; return c - bn_err
 GRNS_HD double ddiv_ru(double a, double b) { double c = a / b; return c + bn_err(c); }
 
 template <bool UP> GRNS_HD double dadd_dir(double a, double b) { return UP ? dadd_ru(a, b) : dadd_rd(a, b); }
+
+// Fractional part x - floor(x) (v_fract_f64 on the device; the same value for finite x >= 0).
+GRNS_HD double fract_d(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fract(v);
+#else
+    return v - floor(v);
+#endif
+}
 
 GRNS_HD uint64_t dbits(double x) {
     union { double d; uint64_t u; } v; v.d = x; return v.u;
@@ -375,6 +385,39 @@ GRNS_HD void psum_pair(Leaf &leaf, double &lo, double &up) {
     }
 }
 
+// One direction of psum_pair: the same tree over leaves leaf.get<I>() returning one double.
+template <int K, int L, int OFF, bool UP>
+struct PTree1 {
+    template <class Leaf>
+    GRNS_HD static double run(Leaf &leaf) {
+        if constexpr (K == 2) {
+            double a = leaf.template get<OFF>();
+            if constexpr (L > 1) return dadd_dir<UP>(a, leaf.template get<OFF + 1>());
+            else return dadd_dir<UP>(a, 0.0);
+        } else {
+            double a = PTree1<K / 2, L, OFF, UP>::run(leaf);
+            double b = 0.0;
+            if constexpr (L > K / 2) b = PTree1<K / 2, L - K / 2, OFF + K / 2, UP>::run(leaf);
+            return dadd_dir<UP>(a, b);
+        }
+    }
+};
+
+template <int N, bool UP, class Leaf>
+GRNS_HD double psum_one(Leaf &leaf) {
+    if constexpr (N >= 4) {
+        return PTree1<TreeWidth<N>::v, N, 0, UP>::run(leaf);
+    } else if constexpr (N == 3) {
+        double a = leaf.template get<0>(), b = leaf.template get<1>(), d = leaf.template get<2>();
+        return dadd_dir<UP>(dadd_dir<UP>(a, b), d);
+    } else if constexpr (N == 2) {
+        double a = leaf.template get<0>(), b = leaf.template get<1>();
+        return dadd_dir<UP>(a, b);
+    } else {
+        return leaf.template get<0>();
+    }
+}
+
 // ---------------------------------------------------------------- wave groups (N lanes per number)
 // A group is G = 2^g >= N consecutive lanes of a wave64, aligned on G; lane gl of a group holds
 // modulus gl.  Cross-lane traffic stays inside the group.
@@ -431,10 +474,111 @@ GRNS_D long long group_sum_ll(const Group &g, long long v) {
     return v;
 }
 
+// ---------------------------------------------------------------- position certificates
+// Sign of the top mixed-radix digit of a canonical x (all the reference's ambiguity branch reads of
+// the conversion, src/rnseval.cuh:62-65) from its position, or -1 when the position leaves it
+// open: F (|F - x/M mod 1| <= err) above 1/m_{n-1} + err without wrapping puts x above M/m_{n-1};
+// a certified representative T (x = T, or M + T when T < 0, |T| < 2^62, M > 2^95) places x below
+// 2^62 < M/m_{n-1} or above M - 2^62 > M/m_{n-1}.
+GRNS_HD int position_digit(const DevCtx *c, double F, bool cert, long long T) {
+    if (F - c->err > c->mr_lim && F + c->err < 1.0) return 1;
+    if (cert) return T < 0 ? 1 : 0;
+    return -1;
+}
+
+// rns_cmp of two numbers from their position classes, 2 when they do not decide it.  Class 1: the
+// CPU-path evaluation is regular (a valid enclosure, F in (tau, 1 - tau)); class 2: x = M + T,
+// T < 0 certified, every reduction canonical (the evaluation is valid through the reference's
+// conversion branch).  Two valid enclosures make rns_cmp the exact sign; class-2 numbers lie above
+// M - 2^62 > M (1 - tau + err), above every class-1 number.
+GRNS_HD int class_order(int kx, long long Tx, int ky, long long Ty) {
+    if (kx == 2 && ky == 2) return (Tx > Ty) - (Tx < Ty);
+    if (kx == 2 && ky == 1) return 1;
+    if (kx == 1 && ky == 2) return -1;
+    return 2;
+}
+
+// Wave-group forms of the certificates: lane gl holds residue gl; the sums run over the group in
+// any order (the certificates verify what they find, and the filter bound holds for any order).
+GRNS_D double group_sum_d(const Group &g, double v) {
+    for (int h = 1; h < g.G; h <<= 1) v += __shfl_xor(v, h, 64);
+    return v;
+}
+GRNS_D unsigned long long group_sum_u64(const Group &g, unsigned long long v) {
+    for (int h = 1; h < g.G; h <<= 1) v += (unsigned long long)__shfl_xor((long long)v, h, 64);
+    return v;
+}
+
+// crt_rep for the group's number with canonical residue di: the representative T of x within 2^62
+// of 0 or M (x = T or M + T), verified against every residue.
+GRNS_D bool group_crt_rep(const DevCtx *c, const Group &g, int di, long long &Ts) {
+    if (!c->big_moduli || c->log2M < 96) return false;
+    const bool act = g.gl < g.L;
+    double a = 0.0;
+    unsigned long long t = 0;
+    if (act) {
+        const int i = g.gl;
+        double p = (double)di * c->dw[i];
+        double q = floor(p * c->dinv[i]);
+        double s = fma(-q, c->dm[i], p) + fma((double)di, c->dw[i], -p);
+        s += (s < 0) ? c->dm[i] : 0.0;
+        s -= (s >= c->dm[i]) ? c->dm[i] : 0.0;
+        a = s * c->dinv[i];
+        t = (unsigned long long)(unsigned)(int)s * c->mi64[i];
+    }
+    const double acc = group_sum_d(g, a);
+    const unsigned long long T = group_sum_u64(g, t) - (unsigned long long)(long long)rint(acc) * c->M64;
+    Ts = (long long)T;
+    if (Ts >= (1ll << 62) || Ts <= -(1ll << 62)) return false;
+    bool ok = true;
+    if (act) {
+        const int m = c->m[g.gl];
+        const long long v = (long long)(int)(Ts >> 32) * (long long)c->c32[g.gl] + (long long)(unsigned)Ts;
+        long long r = rem_trunc(v, m, c->dinv[g.gl]);
+        r += (r < 0) ? m : 0;
+        ok = (r == di);
+    }
+    return group_all(g, ok);
+}
+
+// The comparison filter of the group's number: F (within err of x/M mod 1), whether the residues
+// are canonical and whether every term frac(x_i w_i / m_i) is in (2^-20, 1 - 2^-20).
+GRNS_D double group_filter(const DevCtx *c, const Group &g, int xi, bool &canon, bool &terms) {
+    const bool act = g.gl < g.L;
+    double fi = 0.0;
+    bool cn = true, tm = true;
+    if (act) {
+        const int i = g.gl;
+        const double dx = (double)xi;
+        cn = (unsigned)xi < (unsigned)c->m[i];
+        const double P = dx * c->chi[i];
+        const double e = fma(dx, c->chi[i], -P);
+        const double t = fma(dx, c->clo[i], e);
+        fi = fract_d(P) + t;
+        tm = fabs(fi - 0.5) < 0.5 - 0x1p-20;
+    }
+    const double F = fract_d(group_sum_d(g, fi));
+    canon = group_all(g, cn);
+    terms = group_all(g, tm);
+    return F;
+}
+
+// Sign of the top mixed-radix digit of the group's number from its position (-1: open).
+GRNS_D int group_position_digit(const DevCtx *c, const Group &g, int xi) {
+    bool canon, terms;
+    const double F = group_filter(c, g, xi, canon, terms);
+    if (!canon) return -1;
+    long long T = 0;
+    bool cert = false;
+    if (!(F - c->err > c->mr_lim && F + c->err < 1.0)) cert = group_crt_rep(c, g, xi, T);
+    return position_digit(c, F, cert, T);
+}
+
 // Mixed-radix digits (src/mrc.cuh:31-44), pipelined: at step j the final digit j is broadcast
 // and every higher lane applies the same update the serial loop applies to it.  The inverses of
-// eight steps are requested together, so that the table's memory latency is paid once per eight
-// steps rather than once per step.
+// B steps are requested together, so that the table's memory latency is paid once per B steps
+// rather than once per step (B = 1 keeps the registers of callers that sit in streaming kernels).
+template <int B = 8>
 GRNS_D int group_mrc(const DevCtx *c, const Group &g, int xi) {
     int mr = xi;
     const int n = c->n;
@@ -443,15 +587,15 @@ GRNS_D int group_mrc(const DevCtx *c, const Group &g, int xi) {
     const double dinv = act ? c->dinv[g.gl] : 1.0;
     const int *tab = c->mrc_inv;
     const int steps = g.L - 1;
-    for (int j0 = 0; j0 < steps; j0 += 8) {
-        int inv[8];
+    for (int j0 = 0; j0 < steps; j0 += B) {
+        int inv[B];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < B; k++) {
             const int j = j0 + k;
             inv[k] = (act && j < steps && g.gl > j) ? tab[j * n + g.gl] : 0;
         }
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < B; k++) {
             const int j = j0 + k;
             if (j >= steps) break;  // the same on every lane of the group
             int mrj = bcast_i(mr, g.base + j);
@@ -563,10 +707,14 @@ GRNS_D interval_t group_eval(const DevCtx *c, const Group &g, int xi, int &statu
     out.upp = er_set_d(sumu);
     int mr = -1;
     if (whl != whu) {
-        int d = group_mrc(c, g, xi);
-        mrd = d;
-        has_mrd = true;
-        mr = bcast_i(d, g.base + g.L - 1);
+        // only the sign of the top digit is read: the position of x often fixes it
+        mr = group_position_digit(c, g, xi);
+        if (mr < 0) {
+            int d = group_mrc(c, g, xi);
+            mrd = d;
+            has_mrd = true;
+            mr = bcast_i(d, g.base + g.L - 1);
+        }
     }
     if (mr > 0) { out.upp = c->inv_upp; return out; }
     if (mr == 0) out.low = c->unit_low;
@@ -582,9 +730,54 @@ GRNS_D interval_t group_eval(const DevCtx *c, const Group &g, int xi, int &statu
     return group_eval<FAST>(c, g, xi, status, d, h);
 }
 
+// rns_cmp decided without running the reference's evaluations, or 2: identical residues (0), a
+// zero against a canonical nonzero number, two regular numbers (filter order, then the small- and
+// 64-bit difference certificates), then the position classes.  Each case returns the reference's
+// result (see cmp_lane and class_order).
+GRNS_D int group_cmp_fast(const DevCtx *c, const Group &g, int xi, int yi) {
+    const bool act = g.gl < g.L;
+    if (group_all(g, !act || xi == yi)) return 0;
+    bool cx, tx, cy, ty;
+    const double Fx = group_filter(c, g, xi, cx, tx);
+    const double Fy = group_filter(c, g, yi, cy, ty);
+    if (group_all(g, !act || xi == 0) && cy) return -1;
+    if (group_all(g, !act || yi == 0) && cx) return 1;
+    const bool rx = cx && tx && Fx > c->tau && Fx < 1.0 - c->tau;
+    const bool ry = cy && ty && Fy > c->tau && Fy < 1.0 - c->tau;
+    if (rx && ry) {
+        const double e2 = 2.0 * c->err;
+        if (Fx - Fy > e2) return 1;
+        if (Fy - Fx > e2) return -1;
+        // within M/4 of each other: t = x_0 - y_0 centred, checked against every residue
+        const int m0 = c->m[0];
+        int t = (int)((unsigned)bcast_i(xi, g.base) - (unsigned)bcast_i(yi, g.base));
+        if (t > m0 / 2) t -= m0;
+        else if (t <= -((m0 + 1) / 2)) t += m0;
+        bool ok = true;
+        int d = 0;
+        if (act) {
+            const int m = c->m[g.gl];
+            d = (int)((unsigned)xi - (unsigned)yi);
+            d += (d < 0) ? m : 0;
+            const int tm = t + ((t < 0) ? m : 0);
+            ok = (d == tm) && (2 * (long long)(t < 0 ? -t : t) < (long long)m);
+        }
+        if (group_all(g, ok)) return (t > 0) - (t < 0);
+        long long T;
+        if (group_crt_rep(c, g, d, T)) return (T > 0) - (T < 0);
+        return 2;
+    }
+    long long Tx = 0, Ty = 0;
+    const int kx = rx ? 1 : (cx && tx && group_crt_rep(c, g, xi, Tx) && Tx < 0 ? 2 : 0);
+    const int ky = ry ? 1 : (cy && ty && group_crt_rep(c, g, yi, Ty) && Ty < 0 ? 2 : 0);
+    return class_order(kx, Tx, ky, Ty);
+}
+
 // rns_cmp, src/rnscmp.cuh:20-39 (the digits an evaluation computed are reused by the final
 // mixed-radix comparison: both come from the same conversion of the same residues)
 GRNS_D int group_cmp(const DevCtx *c, const Group &g, int xi, int yi, int &status) {
+    const int f = group_cmp_fast(c, g, xi, yi);
+    if (f != 2) return f;
     int dx = 0, dy = 0;
     bool hx, hy;
     interval_t ex = group_eval<false>(c, g, xi, status, dx, hx);

@@ -152,19 +152,10 @@ GRNS_D bool rare_push(const RareQ &q, bool rare, long long j) {
     return false;
 }
 
-// End of a rare pass: the last block to finish clears the counters.
-GRNS_D void rare_done(const RareQ &q) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned d = atomicAdd(&q.ctr[1], 1u);
-        if (d == gridDim.x - 1) {
-            q.ctr[0] = 0u;
-            q.ctr[2] = 0u;
-            q.ctr[1] = 0u;
-            __threadfence();
-        }
-    }
+// Clears the counters of a rare-item queue for the next call on its stream (one wave, launched
+// after the rare pass).
+static __global__ void k_rare_reset(unsigned *ctr) {
+    if (threadIdx.x < 4) ctr[threadIdx.x] = 0u;
 }
 
 // Marker of an evaluation left to the rare pass after a queue overflow (no result has it).
@@ -235,38 +226,6 @@ __global__ __launch_bounds__(BLOCK) void k_eval(const DevCtx *__restrict__ c, co
     }
 }
 
-// Rare pass of the evaluation: one wave group per queued number (then, after an overflow, per
-// marked output).
-template <bool FAST>
-__global__ __launch_bounds__(BLOCK) void k_eval_rare(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
-                                                     long long smod, long long snum, long long count,
-                                                     interval_t *__restrict__ out, int *__restrict__ status, RareQ q) {
-    Group g = make_group(G, c->n);
-    const int P = 64 / G;
-    const long long gid = (((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6) * P + (threadIdx.x & 63) / G;
-    const long long ngroups = ((long long)gridDim.x * BLOCK >> 6) * P;
-    const unsigned pushed = q.ctr[0];
-    const long long nq = pushed < q.cap ? pushed : q.cap;
-    int st = 0;
-    for (long long k = gid; k < nq; k += ngroups) {
-        const long long j = q.items[k];
-        int xi = g.gl < g.L ? x[j * snum + (long long)g.gl * smod] : 0;
-        interval_t r = group_eval<FAST>(c, g, xi, st);
-        if (g.gl == 0) store_interval(out + j, r);
-    }
-    if (q.ctr[2]) {  // the queue overflowed: the remaining numbers carry the marker
-        for (long long j = gid; j < count; j += ngroups) {
-            const long long e = reinterpret_cast<const long long *>(out + j)[1];
-            if (e != EVAL_MARK) continue;
-            int xi = g.gl < g.L ? x[j * snum + (long long)g.gl * smod] : 0;
-            interval_t r = group_eval<FAST>(c, g, xi, st);
-            if (g.gl == 0) store_interval(out + j, r);
-        }
-    }
-    if (g.gl == 0 && st) atomicOr(status, st);
-    rare_done(q);
-}
-
 // Any n <= 64: one group per number throughout.
 template <bool FAST>
 __global__ __launch_bounds__(BLOCK) void k_eval_group(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
@@ -297,16 +256,9 @@ __global__ __launch_bounds__(BLOCK) void k_eval_group(const DevCtx *__restrict__
 // path's reciprocal reduction of x_i w_i is canonical: its quotient error is below 2^-20.4) and
 // F lies in (tau, 1 - tau) (so its evaluation needs neither mixed-radix conversion nor
 // refinement and is a valid enclosure of x/M).
-GRNS_HD double fract_d(double v) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_fract(v);
-#else
-    return v - floor(v);
-#endif
-}
 
 template <int N, bool REG = true>
-GRNS_HD double filter_frac(const DevCtx *c, const int (&xr)[N], bool &regular, bool &canon) {
+GRNS_HD double filter_frac(const DevCtx *c, const int (&xr)[N], bool &regular, bool &canon, bool &terms) {
     double acc = 0.0, dev = 0.0;
     bool cn = true;
 #pragma unroll
@@ -330,9 +282,41 @@ GRNS_HD double filter_frac(const DevCtx *c, const int (&xr)[N], bool &regular, b
     }
     double F = fract_d(acc);
     canon = cn;
-    regular = REG && cn && dev < 0.5 - 0x1p-20 && F > c->tau && F < 1.0 - c->tau;
+    terms = REG && dev < 0.5 - 0x1p-20;
+    regular = cn && terms && F > c->tau && F < 1.0 - c->tau;
     return F;
 }
+
+template <int N, bool REG = true>
+GRNS_HD double filter_frac(const DevCtx *c, const int (&xr)[N], bool &regular, bool &canon) {
+    bool terms;
+    return filter_frac<N, REG>(c, xr, regular, canon, terms);
+}
+
+// Approximation F of x/M mod 1 for the arg-max keys: sum_i frac(x_i khi_i) + sum_i x_i klo_i, where
+// khi_i (22 significant bits) makes x_i khi_i exact and |x_i klo_i| < 2^9.  For canonical x,
+// |F - x/M mod 1| <= n 2^-38 (the fma chain, |B| <= n 2^9) + n 2^-43 (klo) + 2^-37 (the last
+// sums) < 2^-31 up to n = 64.  Five VALU operations per residue.
+// The sum depends on the residues only mod m_i (w_i/m_i times a multiple of m_i is an integer), so
+// the key holds for non-canonical int32 residues too, which then stand for their canonical value.
+template <int N>
+GRNS_HD double key_frac(const DevCtx *c, const int (&xr)[N]) {
+    double A = 0.0, B = 0.0;
+#pragma unroll
+    for (int i0 = 0; i0 < N; i0 += 8) {
+        const DevCtx *cc = reload_point(c);  // 8 moduli of constants at a time
+        cdouble *kh = const_view(cc->khi + i0);
+        cdouble *kl = const_view(cc->klo + i0);
+#pragma unroll
+        for (int k = 0; k < 8 && i0 + k < N; k++) {
+            double dx = (double)xr[i0 + k];
+            A += fract_d(dx * kh[k]);
+            B = fma(dx, kl[k], B);
+        }
+    }
+    return fract_d(A + B);
+}
+
 
 // Exact sign of x - y for two canonical numbers known to lie within M/4 of each other when their
 // difference is small: t = (x_0 - y_0) reduced into (-m_0/2, m_0/2] is x - y exactly iff
@@ -403,6 +387,40 @@ GRNS_HD int near_sign(const DevCtx *c, const int *px, long long sx, const int *p
     }
     if (!ok) return 2;
     return (Ts > 0) - (Ts < 0);
+}
+
+// The signed representative T of the number with canonical residues d when it lies within 2^62
+// of 0 or of M (x = T or x = M + T), by the construction of near_sign; false when it does not.
+template <int N>
+GRNS_HD bool crt_rep(const DevCtx *c, const int (&d)[N], long long &Ts) {
+    if (!c->big_moduli || c->log2M < 96) return false;  // unique below 2^62 only when M is large
+    unsigned long long T = 0;
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double p = (double)d[i] * c->dw[i];
+        double q = floor(p * c->dinv[i]);
+        double s = fma(-q, c->dm[i], p) + fma((double)d[i], c->dw[i], -p);
+        s += (s < 0) ? c->dm[i] : 0.0;
+        s -= (s >= c->dm[i]) ? c->dm[i] : 0.0;
+        acc = fma(s, c->dinv[i], acc);
+        T += (unsigned long long)(unsigned)(int)s * c->mi64[i];
+    }
+    double K = rint(acc);
+    T -= (unsigned long long)(long long)K * c->M64;
+    Ts = (long long)T;
+    if (Ts >= (1ll << 62) || Ts <= -(1ll << 62)) return false;
+    int th = (int)(Ts >> 32);
+    unsigned tl = (unsigned)Ts;
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        long long v = (long long)th * (long long)c->c32[i] + (long long)tl;
+        long long r = rem_trunc(v, c->m[i], c->dinv[i]);
+        r += (r < 0) ? c->m[i] : 0;
+        ok &= (r == d[i]);
+    }
+    return ok;
 }
 
 // ---------------------------------------------------------------- comparison
@@ -514,40 +532,6 @@ __global__ __launch_bounds__(BLOCK) void k_cmp(const DevCtx *__restrict__ c, con
     }
 }
 
-// Rare pass of the comparison: per queued pair (then, after an overflow, per marked output) one
-// wave group runs the reference algorithm.
-template <int N>
-__global__ __launch_bounds__(BLOCK) void k_cmp_rare(const DevCtx *__restrict__ c, const int *__restrict__ x,
-                                                    long long xsm, long long xsn, const int *__restrict__ y,
-                                                    long long ysm, long long ysn, long long count,
-                                                    int *__restrict__ sign, int *__restrict__ status, RareQ q) {
-    const int G = Pow2Ceil<N>::v;
-    Group g = make_group(G, N);
-    const int P = 64 / G;
-    const long long gid = (((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6) * P + (threadIdx.x & 63) / G;
-    const long long ngroups = ((long long)gridDim.x * BLOCK >> 6) * P;
-    const unsigned pushed = q.ctr[0];
-    const long long nq = pushed < q.cap ? pushed : q.cap;
-    const bool scan = q.ctr[2] != 0;
-    const long long total = nq + (scan ? count : 0);
-    int st = 0;
-    for (long long k = gid; k < total; k += ngroups) {
-        long long j;
-        if (k < nq) {
-            j = q.items[k];
-        } else {
-            j = k - nq;
-            if (sign[j] != 2) continue;
-        }
-        const int xi = g.gl < N ? x[j * xsn + (long long)g.gl * xsm] : 0;
-        const int yi = g.gl < N ? y[j * ysn + (long long)g.gl * ysm] : 0;
-        const int r = group_cmp(c, g, xi, yi, st);
-        if (g.gl == 0) sign[j] = r;
-    }
-    if (g.gl == 0 && st) atomicOr(status, st);
-    rare_done(q);
-}
-
 // Any n <= 64: one group per pair throughout.
 static __global__ __launch_bounds__(BLOCK) void k_cmp_group(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
                                                      long long xsm, long long xsn, const int *__restrict__ y,
@@ -570,6 +554,221 @@ static __global__ __launch_bounds__(BLOCK) void k_cmp_group(const DevCtx *__rest
             }
         }
     }
+}
+
+// ---------------------------------------------------------------- rare passes
+// What the streaming passes leave: numbers near 0 or near M (ambiguous integer parts, refinement)
+// and pairs with such numbers.  A lane first tries to settle its item with exact certificates of
+// the number's position; what remains runs the reference algorithm in wave groups.
+
+// Leaves over the lane's s_i = x_i w_i mod m_i (reciprocal reduction) for the refinement loop:
+// ddiv_ru / ddiv_rd(s_i, m_i) (src/rnseval.cuh:85,93).
+template <bool UP>
+struct DivLeaf {
+    const DevCtx *c;
+    const int *s;
+    template <int I>
+    GRNS_HD double get() {
+        return UP ? ddiv_ru((double)s[I], c->dm[I]) : ddiv_rd((double)s[I], c->dm[I]);
+    }
+};
+// ... and for the main sums: dmul_rd / dmul_ru(s_i, RECIP_RD/RU) (src/rnseval.cuh:46-47).
+struct MulLeaf {
+    const DevCtx *c;
+    const int *s;
+    template <int I>
+    GRNS_HD void get(double &lo, double &up) {
+        lo = dmul_rd((double)s[I], c->rd[I]);
+        up = dmul_ru((double)s[I], c->ru[I]);
+    }
+};
+
+// The refinement loop of rns_eval_compute in one lane (src/rnseval.cuh:79-101), the same steps as
+// group_refine.
+template <int N>
+GRNS_HD void refine_lane(const DevCtx *c, int (&s)[N], double sumu, interval_t &out, int &status) {
+    long long K = 0;
+    int it = 0;
+    while (sumu < c->accuracy) {
+        double kv = (double)c->ref_factor;
+        if (sumu > 0) {
+            double a = (double)(-(ceil_log2_libm(sumu) + 1));
+            kv = a > kv ? a : kv;
+        }
+        if (sumu == 0 || kv > c->log2M || ++it > 4096) {
+            status |= GST_REFINE;
+            out.low.frac = out.upp.frac = __builtin_nan("");
+            out.low.exp = out.upp.exp = 0;
+            return;
+        }
+        const int k = (int)kv;
+        const int *row = c->pow2 + (size_t)k * c->n;
+#pragma unroll
+        for (int i = 0; i < N; i++) s[i] = mulmod_recip(s[i], row[i], c->m[i], c->dinv[i]);
+        DivLeaf<true> lu{c, s};
+        sumu = psum_one<N, true>(lu);
+        sumu -= to_uint(sumu);
+        K += k;
+    }
+    DivLeaf<false> ll{c, s};
+    double suml = psum_one<N, false>(ll);
+    suml -= to_uint(suml);
+    out.low = er_set_d(suml);
+    out.upp = er_set_d(sumu);
+    out.low.exp -= K;
+    out.upp.exp -= K;
+}
+
+// rns_eval_compute[_fast] in one lane for the rare pass.  The ambiguous case (mixed-radix conversion
+// in the reference, which only reads the sign of the top digit) is settled without the conversion
+// when the position of x fixes that digit: x/M above 1/m_{n-1} (key) or x = M + T within 2^62 of M
+// (certificate) make it positive, x = T below 2^62 makes it 0.  Returns false when the position is
+// uncertain (then the cooperative algorithm runs).
+template <int N, bool FAST>
+GRNS_HD bool eval_full(const DevCtx *c, const int (&xr)[N], interval_t &out, int &status) {
+    int nz = 0;
+    bool canon = true;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        nz |= xr[i];
+        canon &= (unsigned)xr[i] < (unsigned)c->m[i];
+    }
+    if (nz == 0) {
+        out.low.frac = 0; out.low.exp = 0; out.upp.frac = 0; out.upp.exp = 0;
+        return true;
+    }
+    int s[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) s[i] = mulmod_recip(xr[i], c->w[i], c->m[i], c->dinv[i]);
+    MulLeaf leaf{c, s};
+    double suml, sumu;
+    psum_pair<N>(leaf, suml, sumu);
+    unsigned whl = to_uint(suml), whu = to_uint(sumu);
+    suml -= whl;
+    sumu -= whu;
+    out.low = er_set_d(suml);
+    out.upp = er_set_d(sumu);
+    if (!FAST && whl != whu) {
+        if (!canon) return false;  // the reference's conversion of non-canonical residues is its own
+        const double F = key_frac<N>(c, xr);
+        long long T = 0;
+        bool cert = false;
+        if (!(F - c->err > c->mr_lim && F + c->err < 1.0)) cert = crt_rep<N>(c, xr, T);
+        const int mr = position_digit(c, F, cert, T);
+        if (mr < 0) return false;
+        if (mr > 0) {
+            out.upp = c->inv_upp;
+            return true;
+        }
+        out.low = c->unit_low;
+    }
+    if (sumu >= c->accuracy) return true;
+    refine_lane<N>(c, s, sumu, out, status);
+    return true;
+}
+
+// Position class of a number for the rare comparisons: 1 when its CPU-path evaluation is regular,
+// 2 when x = M + T with T < 0 certified (canonical, every reduction canonical: the evaluation is a
+// valid enclosure through the reference's conversion branch), 0 otherwise.
+template <int N>
+GRNS_HD int edge_class(const DevCtx *c, const int (&xr)[N], long long &T) {
+    bool reg, canon, terms;
+    filter_frac<N>(c, xr, reg, canon, terms);
+    if (reg) return 1;
+    if (!canon || !terms) return 0;
+    if (crt_rep<N>(c, xr, T) && T < 0) return 2;
+    return 0;
+}
+
+// rns_cmp (src/rnscmp.cuh:20-39) in one lane for a pair the streaming pass left, or 2 when the
+// cooperative algorithm has to run: the filters and certificates of cmp_lane, then the position
+// classes (two valid enclosures make rns_cmp the exact sign), then the reference's own steps with
+// in-lane evaluations, the final mixed-radix order taken from certified positions (it is exact
+// for canonical residues, and so is the order of the certified values).
+template <int N>
+GRNS_HD int cmp_full(const DevCtx *c, const int (&xr)[N], const int (&yr)[N], const int *px, long long sx,
+                    const int *py, long long sy, int &status) {
+    const int r = cmp_lane<N, true>(c, xr, yr, px, sx, py, sy);
+    if (r != 2) return r;
+    long long Tx = 0, Ty = 0;
+    const int kx = edge_class<N>(c, xr, Tx);
+    const int ky = edge_class<N>(c, yr, Ty);
+    const int o = class_order(kx, Tx, ky, Ty);
+    if (o != 2) return o;
+    interval_t ex, ey;
+    if (!eval_full<N, false>(c, xr, ex, status) || !eval_full<N, false>(c, yr, ey, status)) return 2;
+    if (er_ucmp(ex.low, ey.upp) > 0) return 1;
+    if (er_ucmp(ey.low, ex.upp) > 0) return -1;
+    bool eq = true, canon = true;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        eq &= xr[i] == yr[i];
+        canon &= (unsigned)xr[i] < (unsigned)c->m[i] && (unsigned)yr[i] < (unsigned)c->m[i];
+    }
+    if (eq) return 0;
+    if (!canon || !crt_rep<N>(c, xr, Tx) || !crt_rep<N>(c, yr, Ty)) return 2;
+    // x = Tx or M + Tx, y likewise: order the positions
+    const bool hx = Tx < 0, hy = Ty < 0;
+    if (hx != hy) return hx ? 1 : -1;
+    return (Tx > Ty) - (Tx < Ty);
+}
+
+// Item k of a rare pass: the k-th queued index, then (after an overflow) output positions, of
+// which only the marked ones are taken.  -1 for none.
+GRNS_D long long rare_item(const RareQ &q, long long nq, long long k, long long total) {
+    if (k < nq) return q.items[k];
+    return k < total ? k - nq : -1;
+}
+
+// Rare pass of the evaluation: one wave group per item (the group forms settle the ambiguous
+// positions by certificates and run the refinement loop with the whole group).
+template <int N, bool FAST>
+__global__ __launch_bounds__(BLOCK) void k_eval_rare(const DevCtx *__restrict__ c, const int *__restrict__ x,
+                                                     long long smod, long long snum, long long count,
+                                                     interval_t *__restrict__ out, int *__restrict__ status, RareQ q) {
+    const int G = Pow2Ceil<N>::v;
+    Group g = make_group(G, N);
+    const int P = 64 / G;
+    const long long gid = (((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6) * P + (threadIdx.x & 63) / G;
+    const long long ngroups = ((long long)gridDim.x * BLOCK >> 6) * P;
+    const unsigned pushed = q.ctr[0];
+    const long long nq = pushed < q.cap ? pushed : q.cap;
+    const long long total = nq + (q.ctr[2] ? count : 0);
+    int st = 0;
+    for (long long k = gid; k < total; k += ngroups) {
+        const long long j = rare_item(q, nq, k, total);
+        if (k >= nq && reinterpret_cast<const long long *>(out + j)[1] != EVAL_MARK) continue;
+        const int xi = g.gl < N ? x[j * snum + (long long)g.gl * smod] : 0;
+        interval_t r = group_eval<FAST>(c, g, xi, st);
+        if (g.gl == 0) store_interval(out + j, r);
+    }
+    if (g.gl == 0 && st) atomicOr(status, st);
+}
+
+// Rare pass of the comparison: one wave group per item.
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_cmp_rare(const DevCtx *__restrict__ c, const int *__restrict__ x,
+                                                    long long xsm, long long xsn, const int *__restrict__ y,
+                                                    long long ysm, long long ysn, long long count,
+                                                    int *__restrict__ sign, int *__restrict__ status, RareQ q) {
+    const int G = Pow2Ceil<N>::v;
+    Group g = make_group(G, N);
+    const int P = 64 / G;
+    const long long gid = (((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6) * P + (threadIdx.x & 63) / G;
+    const long long ngroups = ((long long)gridDim.x * BLOCK >> 6) * P;
+    const unsigned pushed = q.ctr[0];
+    const long long nq = pushed < q.cap ? pushed : q.cap;
+    const long long total = nq + (q.ctr[2] ? count : 0);
+    int st = 0;
+    for (long long k = gid; k < total; k += ngroups) {
+        const long long j = rare_item(q, nq, k, total);
+        if (k >= nq && sign[j] != 2) continue;
+        const int xi = g.gl < N ? x[j * xsn + (long long)g.gl * xsm] : 0;
+        const int yi = g.gl < N ? y[j * ysn + (long long)g.gl * ysm] : 0;
+        const int r = group_cmp(c, g, xi, yi, st);
+        if (g.gl == 0) sign[j] = r;
+    }
+    if (g.gl == 0 && st) atomicOr(status, st);
 }
 
 // ---------------------------------------------------------------- arg-max (rns_max)
@@ -597,8 +796,8 @@ GRNS_D int group_cmp_exact(const DevCtx *c, const Group &g, const int *__restric
         xb = canon_res(x[b * snum + (long long)g.gl * smod], mi, dinv);
     }
     if (group_all(g, g.gl >= g.L || xa == xb)) return 0;
-    int ma = group_mrc(c, g, xa);
-    int mb = group_mrc(c, g, xb);
+    int ma = group_mrc<1>(c, g, xa);
+    int mb = group_mrc<1>(c, g, xb);
     return group_mrs_cmp(g, ma, mb);
 }
 
@@ -614,10 +813,10 @@ GRNS_D int max_decide(const MaxCand &e, const MaxCand &best) {
 // Exact order (sign of x_a - x_b) settled within one lane: 0 for equal residues, +-1 from the
 // difference certificates when `near` (both numbers known to lie within M/4 of each other), else
 // 2, and a wave group then runs mixed-radix conversion.  pa/pb point at residue 0 of the two
-// numbers, smod apart.  The residues are read from memory: this path is rare and stays out of the
+// numbers, whose residues are sa/sb apart.  The residues are read from memory: this path is rare and stays out of the
 // registers of the streaming loops that call it.
 template <int U>
-GRNS_D int lane_cmp_exact(const DevCtx *c, const int *pa, const int *pb, long long smod, bool near) {
+GRNS_D int lane_cmp_exact(const DevCtx *c, const int *pa, long long sa, const int *pb, long long sb, bool near) {
     const int n = c->n;
     const int m0 = c->m[0];
     const int a0 = pa[0], b0 = pb[0];
@@ -631,7 +830,7 @@ GRNS_D int lane_cmp_exact(const DevCtx *c, const int *pa, const int *pb, long lo
 #pragma unroll U
     for (int i = 1; i < n; i++) {
         const int m = c->m[i];
-        const int ai = pa[i * smod], bi = pb[i * smod];
+        const int ai = pa[i * sa], bi = pb[i * sb];
         eq &= ai == bi;
         canon &= (unsigned)ai < (unsigned)m && (unsigned)bi < (unsigned)m;
         int d = (int)((unsigned)ai - (unsigned)bi);
@@ -642,7 +841,7 @@ GRNS_D int lane_cmp_exact(const DevCtx *c, const int *pa, const int *pb, long lo
     if (eq) return 0;
     if (!near || !canon) return 2;
     if (small) return (t > 0) - (t < 0);
-    return near_sign<U>(c, pa, smod, pb, smod);
+    return near_sign<U>(c, pa, sa, pb, sb);
 }
 
 // Lanes with r == 2 hand their pair (a, b) to the groups of the wave, which order it by mixed-radix
@@ -691,34 +890,11 @@ GRNS_D void max_merge(const DevCtx *c, const int *__restrict__ x, long long smod
     int d = valid ? max_decide(e, best) : 0;
     int r = 0;
     if (d == 2)
-        r = lane_cmp_exact<LANE_UNROLL>(c, x + e.idx * snum, x + best.idx * snum, smod, e.lo >= 0 && best.lo >= 0);
+        r = lane_cmp_exact<LANE_UNROLL>(c, x + e.idx * snum, smod, x + best.idx * snum, smod,
+                                        e.lo >= 0 && best.lo >= 0);
     coop_resolve<N>(c, x, smod, snum, e.idx, best.idx, r);
     bool take = (d == 1) || (d == 2 && (r > 0 || (r == 0 && e.idx < best.idx)));
     if (take) best = e;
-}
-
-// Approximation F of x/M mod 1 for the arg-max keys: sum_i frac(x_i khi_i) + sum_i x_i klo_i, where
-// khi_i (22 significant bits) makes x_i khi_i exact and |x_i klo_i| < 2^9.  For canonical x,
-// |F - x/M mod 1| <= n 2^-38 (the fma chain, |B| <= n 2^9) + n 2^-43 (klo) + 2^-37 (the last
-// sums) < 2^-31 up to n = 64.  Five VALU operations per residue.
-// The sum depends on the residues only mod m_i (w_i/m_i times a multiple of m_i is an integer), so
-// the key holds for non-canonical int32 residues too, which then stand for their canonical value.
-template <int N>
-GRNS_HD double key_frac(const DevCtx *c, const int (&xr)[N]) {
-    double A = 0.0, B = 0.0;
-#pragma unroll
-    for (int i0 = 0; i0 < N; i0 += 8) {
-        const DevCtx *cc = reload_point(c);  // 8 moduli of constants at a time
-        cdouble *kh = const_view(cc->khi + i0);
-        cdouble *kl = const_view(cc->klo + i0);
-#pragma unroll
-        for (int k = 0; k < 8 && i0 + k < N; k++) {
-            double dx = (double)xr[i0 + k];
-            A += fract_d(dx * kh[k]);
-            B = fma(dx, kl[k], B);
-        }
-    }
-    return fract_d(A + B);
 }
 
 template <int N>
@@ -758,7 +934,8 @@ GRNS_D void block_max_reduce(const DevCtx *c, const int *__restrict__ x, long lo
 template <int N>
 __global__ __launch_bounds__(BLOCK) void k_max_partial(const DevCtx *__restrict__ c, const int *__restrict__ x,
                                                        long long smod, long long snum, long long count,
-                                                       long long chunk, MaxCand *__restrict__ part) {
+                                                       long long chunk, MaxCand *__restrict__ part,
+                                                       int *__restrict__ part_res) {
     __shared__ MaxCand sh[BLOCK];
     long long start = (long long)blockIdx.x * chunk;
     long long end = start + chunk < count ? start + chunk : count;
@@ -778,6 +955,9 @@ __global__ __launch_bounds__(BLOCK) void k_max_partial(const DevCtx *__restrict_
     }
     block_max_reduce<N>(c, x, smod, snum, best, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = best;
+    // a copy of the winner's residues, so that stage 2 reads them from one contiguous line
+    if (threadIdx.x < (unsigned)N && best.idx >= 0)
+        part_res[(long long)blockIdx.x * N + threadIdx.x] = x[best.idx * snum + (long long)threadIdx.x * smod];
 }
 
 // Pivot order for stage 2: the larger lower key bound, then the lower index.
@@ -787,55 +967,64 @@ GRNS_D bool pivot_before(const MaxCand &a, const MaxCand &b) {
     return a.lo > b.lo || (a.lo == b.lo && a.idx < b.idx);
 }
 
-constexpr int MAX_FINAL_PER_THREAD = 32;  // stage 2 takes at most BLOCK * 32 candidates
+constexpr int FBLOCK = 1024;             // threads of the stage-2 block
+constexpr int MAX_FINAL_PER_THREAD = 8;  // stage 2 takes at most FBLOCK * 8 candidates
 
 // Stage 2 (one block): the exact maximum of the per-block candidates, by rounds.  The live
 // candidate with the largest lower key bound is the pivot; every other live candidate is ordered
-// against it (keys when they separate, else exactly: in-lane certificates, then wave groups).
-// Candidates below or equal to the pivot leave (equal ones lower the tie index); those above
-// stay live for the next round.  When none is above, the pivot's value is the maximum and the
-// tie index is the lowest index holding it.
+// against it (keys when they separate, else exactly: in-lane certificates on the residue copies
+// stage 1 made, then wave groups).  Candidates below or equal to the pivot leave (equal ones lower
+// the tie index); those above stay live for the next round.  When none is above, the pivot's value
+// is the maximum and the tie index is the lowest index holding it.
 template <int N>
-__global__ __launch_bounds__(BLOCK) void k_max_final(const DevCtx *__restrict__ c, const int *__restrict__ x,
-                                                     long long smod, long long snum, const MaxCand *__restrict__ part,
-                                                     int nparts, int *__restrict__ out, long long *__restrict__ out_idx) {
-    __shared__ MaxCand sh[BLOCK];
+__global__ __launch_bounds__(FBLOCK) void k_max_final(const DevCtx *__restrict__ c, const int *__restrict__ x,
+                                                      long long smod, long long snum,
+                                                      const MaxCand *__restrict__ part,
+                                                      const int *__restrict__ part_res, int nparts,
+                                                      int *__restrict__ out, long long *__restrict__ out_idx) {
+    __shared__ MaxCand sh[FBLOCK];
+    __shared__ int shk[FBLOCK];
     __shared__ unsigned long long s_tie;
     const int T = threadIdx.x;
-    const int per = (nparts + BLOCK - 1) / BLOCK;
+    const int per = (nparts + FBLOCK - 1) / FBLOCK;
     unsigned live = 0;
     for (int t = 0; t < per; t++) {
-        int k = T + t * BLOCK;
+        int k = T + t * FBLOCK;
         if (k < nparts && part[k].idx >= 0) live |= 1u << t;
     }
     long long win = -1;
     for (;;) {
         MaxCand b;
         b.idx = -1; b.lo = -2.0; b.hi = -2.0;
+        int bk = -1;
         for (int t = 0; t < per; t++)
             if (live >> t & 1) {
-                MaxCand e = part[T + t * BLOCK];
-                if (pivot_before(e, b)) b = e;
+                MaxCand e = part[T + t * FBLOCK];
+                if (pivot_before(e, b)) { b = e; bk = T + t * FBLOCK; }
             }
         sh[T] = b;
+        shk[T] = bk;
         if (T == 0) s_tie = ~0ull;
         __syncthreads();
-        for (int h = BLOCK / 2; h >= 1; h >>= 1) {
-            if (T < h && pivot_before(sh[T + h], sh[T])) sh[T] = sh[T + h];
+        for (int h = FBLOCK / 2; h >= 1; h >>= 1) {
+            if (T < h && pivot_before(sh[T + h], sh[T])) { sh[T] = sh[T + h]; shk[T] = shk[T + h]; }
             __syncthreads();
         }
         const MaxCand P = sh[0];
+        const int pk = shk[0];
         if (P.idx < 0) break;  // uniform: every candidate has left
+        const int *pres = part_res + (long long)pk * N;
         bool above = false;
         unsigned long long tie = (unsigned long long)P.idx;
         for (int t = 0; t < per; t++) {  // uniform trip count: the wave groups may be needed
             const bool mine = live >> t & 1;
+            const int k = T + t * FBLOCK;
             MaxCand e;
             e.idx = -1; e.lo = -1.0; e.hi = -1.0;
-            if (mine) e = part[T + t * BLOCK];
+            if (mine) e = part[k];
             int r = -1;
             if (mine && e.idx != P.idx && e.hi >= P.lo)
-                r = lane_cmp_exact<N>(c, x + e.idx * snum, x + P.idx * snum, smod, e.lo >= 0 && P.lo >= 0);
+                r = lane_cmp_exact<16>(c, part_res + (long long)k * N, 1, pres, 1, e.lo >= 0 && P.lo >= 0);
             coop_resolve<N>(c, x, smod, snum, e.idx, P.idx, r);
             if (mine) {
                 if (r > 0) {
@@ -854,7 +1043,7 @@ __global__ __launch_bounds__(BLOCK) void k_max_final(const DevCtx *__restrict__ 
         }
         __syncthreads();  // s_tie and sh are rewritten by the next round
     }
-    if (T < (unsigned)c->n && win >= 0) out[T] = x[win * snum + (long long)T * smod];
+    if (T < N && win >= 0) out[T] = x[win * snum + (long long)T * smod];
     if (T == 0) *out_idx = win;
 }
 

@@ -120,6 +120,7 @@ grns_status build_constants(grns_ctx *c, const int *moduli, int n) {
     while (tau < 4 * h.accuracy) tau *= 2;
     h.tau = tau;
     h.err = 0x1p-30;
+    h.mr_lim = std::nextafter(1.0 / (double)moduli[n - 1], 2.0);
     h.big_moduli = 1;
     for (int i = 0; i < n; i++) h.big_moduli &= (moduli[i] >= (1 << 22));
     // RNS_EVAL_UNIT / INV_UNIT: M rounded down to 53 bits, then 1/Md and 1 - 1/Md rounded up and
@@ -228,10 +229,11 @@ grns_status rare_queue(grns_ctx *c, void *stream, RareQ &q) {
     return GRNS_OK;
 }
 
-// Grid of a rare pass: enough wave groups for about one number in eight, at most 2048 blocks.
+// Grid of a rare pass (one wave group per item): about one group per item up to 4096 items, then
+// 2048 blocks (16384 groups of 32 lanes) that the items share.
 long long rare_grid(size_t count) {
-    long long b = (long long)((count + 63) / 64);
-    if (b < 1) b = 1;
+    long long b = (long long)((count + 7) / 8);
+    if (b < 64) b = 64;
     if (b > 2048) b = 2048;
     return b;
 }
@@ -367,15 +369,16 @@ grns_status grns_eval(grns_ctx *c, const int *x, long long smod, long long snum,
     case NN: {                                                                                                   \
         grns_status qs = rare_queue(c, stream, q);                                                               \
         if (qs != GRNS_OK) return qs;                                                                            \
-        const int G = groups_for(c->n);                                                                          \
         dim3 rg((unsigned)rare_grid(count));                                                                     \
         if (fast) hipLaunchKernelGGL((k_eval<NN, true>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, o, q); \
         else hipLaunchKernelGGL((k_eval<NN, false>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, o, q);     \
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                                \
-        if (fast) hipLaunchKernelGGL((k_eval_rare<true>), rg, blk, 0, S(stream), c->d, G, x, smod, snum, cnt, o,   \
+        if (fast) hipLaunchKernelGGL((k_eval_rare<NN, true>), rg, blk, 0, S(stream), c->d, x, smod, snum, cnt, o,  \
                                      c->d_status, q);                                                            \
-        else hipLaunchKernelGGL((k_eval_rare<false>), rg, blk, 0, S(stream), c->d, G, x, smod, snum, cnt, o,       \
+        else hipLaunchKernelGGL((k_eval_rare<NN, false>), rg, blk, 0, S(stream), c->d, x, smod, snum, cnt, o,      \
                                 c->d_status, q);                                                                 \
+        if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                                \
+        hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), q.ctr);                                \
         return launch_check();                                                                                   \
     }
     switch (c->n) {
@@ -414,6 +417,8 @@ grns_status grns_cmp(grns_ctx *c, const int *x, long long xsm, long long xsn, co
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                                  \
         hipLaunchKernelGGL((k_cmp_rare<NN>), rg, blk, 0, S(stream), c->d, x, xsm, xsn, y, ysm, ysn, cnt, sign,       \
                            c->d_status, q);                                                                        \
+        if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                                  \
+        hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), q.ctr);                                  \
         return launch_check();                                                                                     \
     }
     switch (c->n) {
@@ -466,7 +471,7 @@ grns_status grns_scale2pow(grns_ctx *c, const int *x, long long smod, long long 
 // creation), each over a contiguous chunk that is a multiple of the block size.
 static void max_shape(const grns_ctx *c, size_t count, long long &nblocks, long long &chunk) {
     long long target = c->max_blocks;
-    if (target > (long long)BLOCK * MAX_FINAL_PER_THREAD) target = (long long)BLOCK * MAX_FINAL_PER_THREAD;
+    if (target > (long long)FBLOCK * MAX_FINAL_PER_THREAD) target = (long long)FBLOCK * MAX_FINAL_PER_THREAD;
     if (target < 1) target = 1;
     long long per = (long long)((count + target - 1) / target);
     per = ((per + BLOCK - 1) / BLOCK) * BLOCK;
@@ -476,11 +481,16 @@ static void max_shape(const grns_ctx *c, size_t count, long long &nblocks, long 
     if (nblocks < 1) nblocks = 1;
 }
 
+// Workspace of the arg-max: the per-block candidates, then a copy of each candidate's residues.
+static size_t max_ws_bytes(const grns_ctx *c, long long nb) {
+    return (size_t)nb * sizeof(MaxCand) + (size_t)nb * c->n * sizeof(int);
+}
+
 size_t grns_max_workspace_size(const grns_ctx *c, size_t count) {
     if (!c) return 0;
     long long nb, ch;
     max_shape(c, count, nb, ch);
-    return (size_t)nb * sizeof(MaxCand);
+    return max_ws_bytes(c, nb);
 }
 
 grns_status grns_max(grns_ctx *c, const int *x, long long smod, long long snum, size_t count, int *out_residues,
@@ -489,17 +499,19 @@ grns_status grns_max(grns_ctx *c, const int *x, long long smod, long long snum, 
     if (!c || !x || !out_residues || !out_idx || count == 0) return GRNS_ERR_ARG;
     long long nb, ch;
     max_shape(c, count, nb, ch);
-    if (!workspace || ws_bytes < (size_t)nb * sizeof(MaxCand)) return GRNS_ERR_ARG;
+    if (!workspace || ws_bytes < max_ws_bytes(c, nb)) return GRNS_ERR_ARG;
     DeviceGuard guard(c->device);
     MaxCand *part = reinterpret_cast<MaxCand *>(workspace);
-    dim3 grid((unsigned)nb), blk(BLOCK);
+    int *part_res = reinterpret_cast<int *>(part + nb);
+    dim3 grid((unsigned)nb), blk(BLOCK), fblk(FBLOCK);
     long long cnt = (long long)count;
 #define GRNS_MAX_CASE(NN)                                                                                    \
     case NN:                                                                                                 \
-        hipLaunchKernelGGL((k_max_partial<NN>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, ch, part); \
+        hipLaunchKernelGGL((k_max_partial<NN>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, ch, part,  \
+                           part_res);                                                                        \
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                            \
-        hipLaunchKernelGGL((k_max_final<NN>), dim3(1), blk, 0, S(stream), c->d, x, smod, snum, part, (int)nb, \
-                           out_residues, out_idx);                                                           \
+        hipLaunchKernelGGL((k_max_final<NN>), dim3(1), fblk, 0, S(stream), c->d, x, smod, snum, part,        \
+                           part_res, (int)nb, out_residues, out_idx);                                        \
         break;
     switch (c->n) {
         GRNS_MAX_CASE(4)
@@ -590,6 +602,34 @@ double host_filter_n(const grns_ctx *c, const int *x, int *regular, int *canon) 
     *canon = cn;
     return F;
 }
+// The device constants with the host copies of the tables (for the host runs of device routines).
+DevCtx host_view(const grns_ctx *c) {
+    DevCtx h = c->h;
+    h.pow2 = c->pow2.data();
+    h.mrc_inv = c->mrc.data();
+    return h;
+}
+template <int N>
+int host_eval_full_n(const grns_ctx *c, const int *x, grns_interval *out, int fast, int *status) {
+    const DevCtx h = host_view(c);
+    int xr[N];
+    for (int i = 0; i < N; i++) xr[i] = x[i];
+    interval_t r;
+    int st = 0;
+    bool ok = fast ? eval_full<N, true>(&h, xr, r, st) : eval_full<N, false>(&h, xr, r, st);
+    out->low.frac = r.low.frac; out->low.exp = r.low.exp;
+    out->upp.frac = r.upp.frac; out->upp.exp = r.upp.exp;
+    *status = st;
+    return ok ? 1 : 0;
+}
+template <int N>
+int host_cmp_full_n(const grns_ctx *c, const int *x, const int *y) {
+    const DevCtx h = host_view(c);
+    int xr[N], yr[N];
+    for (int i = 0; i < N; i++) { xr[i] = x[i]; yr[i] = y[i]; }
+    int st = 0;
+    return cmp_full<N>(&h, xr, yr, x, 1, y, 1, st);
+}
 template <int N>
 double host_key_n(const grns_ctx *c, const int *x) {
     int xr[N];
@@ -630,6 +670,29 @@ double grns_debug_max_key(const grns_ctx *c, const int *x) {
         case 32: return host_key_n<32>(c, x);
         case 64: return host_key_n<64>(c, x);
         default: return -1.0;
+    }
+}
+/* Rare-pass evaluation of one number in one lane: 1 when final (out, status set), 0 when the
+ * cooperative path runs, -1 if n is unsupported. */
+int grns_debug_eval_full(const grns_ctx *c, const int *x, grns_interval *out, int fast, int *status) {
+    switch (c->n) {
+        case 4: return host_eval_full_n<4>(c, x, out, fast, status);
+        case 8: return host_eval_full_n<8>(c, x, out, fast, status);
+        case 16: return host_eval_full_n<16>(c, x, out, fast, status);
+        case 32: return host_eval_full_n<32>(c, x, out, fast, status);
+        case 64: return host_eval_full_n<64>(c, x, out, fast, status);
+        default: return -1;
+    }
+}
+/* Rare-pass comparison in one lane: 1, 0, -1, or 2 when the cooperative path runs. */
+int grns_debug_cmp_full(const grns_ctx *c, const int *x, const int *y) {
+    switch (c->n) {
+        case 4: return host_cmp_full_n<4>(c, x, y);
+        case 8: return host_cmp_full_n<8>(c, x, y);
+        case 16: return host_cmp_full_n<16>(c, x, y);
+        case 32: return host_cmp_full_n<32>(c, x, y);
+        case 64: return host_cmp_full_n<64>(c, x, y);
+        default: return -2;
     }
 }
 /* In-lane decision of the comparison kernel: 1, 0, -1, or 2 when the cooperative path runs. */

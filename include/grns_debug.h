@@ -14,6 +14,8 @@ int grns_debug_eval_hot(const grns_ctx *ctx, const int *x, grns_interval *out, i
 double grns_debug_filter(const grns_ctx *ctx, const int *x, int *regular, int *canon);
 int grns_debug_cmp_lane(const grns_ctx *ctx, const int *x, const int *y);
 double grns_debug_max_key(const grns_ctx *ctx, const int *x);
+int grns_debug_eval_full(const grns_ctx *ctx, const int *x, grns_interval *out, int fast, int *status);
+int grns_debug_cmp_full(const grns_ctx *ctx, const int *x, const int *y);
 int grns_debug_ceil_log2(double s);
 double grns_debug_mulmod_recip_d(double x, double y, double m, double d);
 int grns_debug_mulmod_recip(int x, int y, int m, double d);
