@@ -97,20 +97,29 @@ GRNS_D void store_interval(interval_t *p, const interval_t &v) {
 // scalar base and a 32-bit vector offset.
 typedef const __attribute__((address_space(1))) int global_int;
 
+// Residues of number j into registers.  LD_MM: modulus-major (snum == 1) and j < 2^29, checked by
+// the launcher: a wave-uniform row pointer stepped from modulus to modulus plus one 32-bit byte
+// offset, so each load is a scalar-base global load (the row pointer is recomputed per modulus
+// rather than N of them being held in scalar registers).  LD_ANY: any strides.  LD_AUTO: the
+// modulus-major form when the number allows it.
+enum LoadMode { LD_ANY = 0, LD_MM = 1, LD_AUTO = 2 };
+
 template <int N>
-GRNS_D void load_number(const int *__restrict__ x, long long smod, long long snum, long long j, int (&xr)[N]) {
-    if (snum == 1 && j < (1ll << 29)) {
-        // modulus-major: a wave-uniform row pointer stepped from modulus to modulus plus one 32-bit
-        // byte offset, so each load is a scalar-base global load; the row pointer is recomputed
-        // per modulus (reload_point) rather than N of them being held in scalar registers
-        const unsigned off = lane_offset((unsigned)j * 4u);
-        const long long step = smod * 4;
-        long long row = reinterpret_cast<long long>(x);
+GRNS_D void load_number_mm(const int *__restrict__ x, long long smod, long long j, int (&xr)[N]) {
+    const unsigned off = lane_offset((unsigned)j * 4u);
+    const long long step = smod * 4;
+    long long row = reinterpret_cast<long long>(x);
 #pragma unroll
-        for (int i = 0; i < N; i++) {
-            xr[i] = __builtin_nontemporal_load(reinterpret_cast<const global_int *>(row + off));
-            row = uniform_point(row + step);
-        }
+    for (int i = 0; i < N; i++) {
+        xr[i] = __builtin_nontemporal_load(reinterpret_cast<const global_int *>(row + off));
+        row = uniform_point(row + step);
+    }
+}
+
+template <int N, int MODE = LD_AUTO>
+GRNS_D void load_number(const int *__restrict__ x, long long smod, long long snum, long long j, int (&xr)[N]) {
+    if (MODE == LD_MM || (MODE == LD_AUTO && snum == 1 && j < (1ll << 29))) {
+        load_number_mm<N>(x, smod, j, xr);
         return;
     }
     const int *p = x + j * snum;
@@ -203,7 +212,7 @@ GRNS_D void eval_step(const DevCtx *__restrict__ c, const int *__restrict__ x, l
 
 // Plain per-call evaluation, streaming pass: one number per lane; numbers that need the
 // cooperative algorithm go to the rare-item queue.
-template <int N, bool FAST>
+template <int N, bool FAST, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_eval(const DevCtx *__restrict__ c, const int *__restrict__ x, long long smod,
                                                 long long snum, long long count, interval_t *__restrict__ out,
                                                 RareQ q) {
@@ -212,7 +221,7 @@ __global__ __launch_bounds__(BLOCK) void k_eval(const DevCtx *__restrict__ c, co
         bool rare = false;
         if (j < count) {
             int xr[N];
-            load_number<N>(x, smod, snum, j, xr);
+            load_number<N, MODE>(x, smod, snum, j, xr);
             interval_t r;
             if (eval_hot<N, FAST>(c, xr, r)) store_interval(out + j, r);
             else rare = true;
@@ -257,34 +266,44 @@ __global__ __launch_bounds__(BLOCK) void k_eval_group(const DevCtx *__restrict__
 // F lies in (tau, 1 - tau) (so its evaluation needs neither mixed-radix conversion nor
 // refinement and is a valid enclosure of x/M).
 
+// One term of the filter: frac(x_i w_i / m_i) within 2^-42 (x_i khi_i is exact, |x_i klo_i| < 2^9
+// is added with one rounding of at most 2^-44, klo_i is within 2^-74 of w_i/m_i - khi_i).
+GRNS_HD double filter_term(double dx, double kh, double kl) { return fract_d(fma(dx, kl, fract_d(dx * kh))); }
+
+// The flags of one number: residues canonical (x_i | (m_i - 1 - x_i) >= 0 for all i) and every
+// term in (2^-20, 1 - 2^-20).
+struct FilterAcc {
+    double acc = 0.0, lo = 1.0, hi = 0.0;
+    int bad = 0;
+    GRNS_HD void add(int xi, int m, double kh, double kl) {
+        bad |= xi | (m - 1 - xi);
+        const double f = filter_term((double)xi, kh, kl);
+        lo = fmin(lo, f);
+        hi = fmax(hi, f);
+        acc += f;
+    }
+    GRNS_HD double frac(const DevCtx *c, bool &regular, bool &canon, bool &terms) const {
+        const double F = fract_d(acc);
+        canon = bad >= 0;
+        terms = lo > 0x1p-20 && hi < 1.0 - 0x1p-20;
+        regular = canon && terms && F > c->tau && F < 1.0 - c->tau;
+        return F;
+    }
+};
+
 template <int N, bool REG = true>
 GRNS_HD double filter_frac(const DevCtx *c, const int (&xr)[N], bool &regular, bool &canon, bool &terms) {
-    double acc = 0.0, dev = 0.0;
-    bool cn = true;
+    FilterAcc a;
 #pragma unroll
     for (int i0 = 0; i0 < N; i0 += 8) {
         const DevCtx *cc = reload_point(c);  // 8 moduli of constants at a time
-        cdouble *ch = const_view(cc->chi + i0);
-        cdouble *cl = const_view(cc->clo + i0);
+        cdouble *kh = const_view(cc->khi + i0);
+        cdouble *kl = const_view(cc->klo + i0);
         cint *cm = const_view(cc->m + i0);
 #pragma unroll
-        for (int k = 0; k < 8 && i0 + k < N; k++) {
-            const int xi = xr[i0 + k];
-            double dx = (double)xi;
-            cn &= (unsigned)xi < (unsigned)cm[k];
-            double P = dx * ch[k];
-            double e = fma(dx, ch[k], -P);
-            double t = fma(dx, cl[k], e);
-            double fi = fract_d(P) + t;
-            if (REG) dev = fmax(dev, fabs(fi - 0.5));
-            acc += fi;
-        }
+        for (int k = 0; k < 8 && i0 + k < N; k++) a.add(xr[i0 + k], cm[k], kh[k], kl[k]);
     }
-    double F = fract_d(acc);
-    canon = cn;
-    terms = REG && dev < 0.5 - 0x1p-20;
-    regular = cn && terms && F > c->tau && F < 1.0 - c->tau;
-    return F;
+    return a.frac(c, regular, canon, terms);
 }
 
 template <int N, bool REG = true>
@@ -431,9 +450,11 @@ GRNS_HD bool crt_rep(const DevCtx *c, const int (&d)[N], long long &Ts) {
 template <int N, bool NEAR = true>
 GRNS_HD int cmp_lane(const DevCtx *c, const int (&xr)[N], const int (&yr)[N], const int *px, long long sx,
                      const int *py, long long sy) {
-    bool rx, ry, cx, cy;
-    double fx = filter_frac<N>(c, xr, rx, cx);
-    double fy = filter_frac<N>(c, yr, ry, cy);
+    bool rx, ry, cx, cy, tx, ty;
+    // one pass per number, each with its own scalar loads of the constants (sharing them would keep
+    // them live across both passes)
+    const double fx = filter_frac<N>(c, xr, rx, cx, tx);
+    const double fy = filter_frac<N>(c, yr, ry, cy, ty);
     if (!(rx && ry)) {
         // Outside the regular case the reference's evaluations have to be run, except in two cases
         // whose rns_cmp result does not depend on them: identical residues (the same interval twice,
@@ -514,7 +535,7 @@ GRNS_D void cmp_step(const DevCtx *__restrict__ c, const int *__restrict__ x, lo
 
 // Plain per-call comparison, streaming pass: one pair per lane, no cooperative code (so the loop
 // keeps its registers); pairs the lane cannot settle go to the rare-item queue.
-template <int N>
+template <int N, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_cmp(const DevCtx *__restrict__ c, const int *__restrict__ x, long long xsm,
                                                long long xsn, const int *__restrict__ y, long long ysm, long long ysn,
                                                long long count, int *__restrict__ sign, RareQ q) {
@@ -523,8 +544,8 @@ __global__ __launch_bounds__(BLOCK) void k_cmp(const DevCtx *__restrict__ c, con
         int res = 0;
         if (j < count) {
             int xr[N], yr[N];
-            load_number<N>(x, xsm, xsn, j, xr);
-            load_number<N>(y, ysm, ysn, j, yr);
+            load_number<N, MODE>(x, xsm, xsn, j, xr);
+            load_number<N, MODE>(y, ysm, ysn, j, yr);
             res = cmp_lane<N, false>(c, xr, yr, x + j * xsn, xsm, y + j * ysn, ysm);
             if (res != 2) sign[j] = res;
         }
@@ -931,7 +952,7 @@ GRNS_D void block_max_reduce(const DevCtx *c, const int *__restrict__ x, long lo
 }
 
 // Stage 1: each block reduces a contiguous chunk to one candidate.
-template <int N>
+template <int N, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_max_partial(const DevCtx *__restrict__ c, const int *__restrict__ x,
                                                        long long smod, long long snum, long long count,
                                                        long long chunk, MaxCand *__restrict__ part,
@@ -948,7 +969,7 @@ __global__ __launch_bounds__(BLOCK) void k_max_partial(const DevCtx *__restrict_
         bool valid = j < end;
         if (valid) {
             int xr[N];
-            load_number<N>(x, smod, snum, j, xr);
+            load_number<N, MODE>(x, smod, snum, j, xr);
             e = max_key<N>(c, xr, j);
         }
         max_merge<N>(c, x, smod, snum, best, e, valid);

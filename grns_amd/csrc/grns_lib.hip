@@ -245,11 +245,11 @@ int max_grid_for(const grns_ctx *c) {
         return 2048;
     hipError_t e = hipErrorInvalidValue;
     switch (c->n) {
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<4>, BLOCK, 0); break;
-        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<8>, BLOCK, 0); break;
-        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<16>, BLOCK, 0); break;
-        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<32>, BLOCK, 0); break;
-        case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<64>, BLOCK, 0); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<4, LD_MM>, BLOCK, 0); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<8, LD_MM>, BLOCK, 0); break;
+        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<16, LD_MM>, BLOCK, 0); break;
+        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<32, LD_MM>, BLOCK, 0); break;
+        case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_max_partial<64, LD_MM>, BLOCK, 0); break;
         default: break;
     }
     if (e != hipSuccess || per < 1) return 2048;
@@ -365,13 +365,23 @@ grns_status grns_eval(grns_ctx *c, const int *x, long long smod, long long snum,
     dim3 grid((unsigned)grid_elems(count)), blk(BLOCK);
     long long cnt = (long long)count;
     RareQ q;
+    const bool mm = snum == 1 && count <= ((size_t)1 << 29);
 #define GRNS_EVAL_CASE(NN)                                                                                       \
     case NN: {                                                                                                   \
         grns_status qs = rare_queue(c, stream, q);                                                               \
         if (qs != GRNS_OK) return qs;                                                                            \
         dim3 rg((unsigned)rare_grid(count));                                                                     \
-        if (fast) hipLaunchKernelGGL((k_eval<NN, true>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, o, q); \
-        else hipLaunchKernelGGL((k_eval<NN, false>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, o, q);     \
+        if (mm) {                                                                                                \
+            if (fast) hipLaunchKernelGGL((k_eval<NN, true, LD_MM>), grid, blk, 0, S(stream), c->d, x, smod, snum,  \
+                                         cnt, o, q);                                                             \
+            else hipLaunchKernelGGL((k_eval<NN, false, LD_MM>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, \
+                                    o, q);                                                                       \
+        } else {                                                                                                 \
+            if (fast) hipLaunchKernelGGL((k_eval<NN, true, LD_ANY>), grid, blk, 0, S(stream), c->d, x, smod, snum, \
+                                         cnt, o, q);                                                             \
+            else hipLaunchKernelGGL((k_eval<NN, false, LD_ANY>), grid, blk, 0, S(stream), c->d, x, smod, snum,     \
+                                    cnt, o, q);                                                                  \
+        }                                                                                                        \
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                                \
         if (fast) hipLaunchKernelGGL((k_eval_rare<NN, true>), rg, blk, 0, S(stream), c->d, x, smod, snum, cnt, o,  \
                                      c->d_status, q);                                                            \
@@ -408,12 +418,16 @@ grns_status grns_cmp(grns_ctx *c, const int *x, long long xsm, long long xsn, co
     dim3 grid((unsigned)grid_elems(count)), blk(BLOCK);
     long long cnt = (long long)count;
     RareQ q;
+    const bool mm = xsn == 1 && ysn == 1 && count <= ((size_t)1 << 29);
 #define GRNS_CMP_CASE(NN)                                                                                          \
     case NN: {                                                                                                     \
         grns_status qs = rare_queue(c, stream, q);                                                                 \
         if (qs != GRNS_OK) return qs;                                                                              \
         dim3 rg((unsigned)rare_grid(count));                                                                       \
-        hipLaunchKernelGGL((k_cmp<NN>), grid, blk, 0, S(stream), c->d, x, xsm, xsn, y, ysm, ysn, cnt, sign, q);        \
+        if (mm) hipLaunchKernelGGL((k_cmp<NN, LD_MM>), grid, blk, 0, S(stream), c->d, x, xsm, xsn, y, ysm, ysn, cnt,  \
+                                   sign, q);                                                                       \
+        else hipLaunchKernelGGL((k_cmp<NN, LD_ANY>), grid, blk, 0, S(stream), c->d, x, xsm, xsn, y, ysm, ysn, cnt,     \
+                                sign, q);                                                                          \
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                                  \
         hipLaunchKernelGGL((k_cmp_rare<NN>), rg, blk, 0, S(stream), c->d, x, xsm, xsn, y, ysm, ysn, cnt, sign,       \
                            c->d_status, q);                                                                        \
@@ -503,12 +517,15 @@ grns_status grns_max(grns_ctx *c, const int *x, long long smod, long long snum, 
     DeviceGuard guard(c->device);
     MaxCand *part = reinterpret_cast<MaxCand *>(workspace);
     int *part_res = reinterpret_cast<int *>(part + nb);
+    const bool mm = snum == 1 && count <= ((size_t)1 << 29);
     dim3 grid((unsigned)nb), blk(BLOCK), fblk(FBLOCK);
     long long cnt = (long long)count;
 #define GRNS_MAX_CASE(NN)                                                                                    \
     case NN:                                                                                                 \
-        hipLaunchKernelGGL((k_max_partial<NN>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, ch, part,  \
-                           part_res);                                                                        \
+        if (mm) hipLaunchKernelGGL((k_max_partial<NN, LD_MM>), grid, blk, 0, S(stream), c->d, x, smod, snum, \
+                                   cnt, ch, part, part_res);                                                 \
+        else hipLaunchKernelGGL((k_max_partial<NN, LD_ANY>), grid, blk, 0, S(stream), c->d, x, smod, snum,    \
+                                cnt, ch, part, part_res);                                                    \
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                            \
         hipLaunchKernelGGL((k_max_final<NN>), dim3(1), fblk, 0, S(stream), c->d, x, smod, snum, part,        \
                            part_res, (int)nb, out_residues, out_idx);                                        \
