@@ -105,6 +105,7 @@ struct DevCtx {
     double tau;        // fraction window outside of which no evaluation needs MRC or refinement
     double err;        // bound on |approx fraction - x/M mod 1| of the filter sums
     double mr_lim;     // 1 / m_{n-1} rounded up: x/M above it makes the top mixed-radix digit positive
+    int m_min;         // smallest modulus
     er_t unit_low, unit_upp, inv_low, inv_upp;
     int m[MAXN];
     int w[MAXN];                 // RNS_PART_MODULI_PRODUCT_INVERSE
@@ -262,6 +263,36 @@ GRNS_HD double mulmod_recip_d(double dx, double dy, double dm, double d) {
     double k = floor(fma(s, 0x1p-32, 0.5));
     return fma(-k, 4294967296.0, s);
 }
+// The same before the int32 wrap: callers that treat |s| >= 2^31 (where the wrap would apply)
+// as a rare case skip its three operations.
+GRNS_HD double mulmod_recip_d_nowrap(double dx, double dy, double dm, double d) {
+    double p = dx * dy;
+    double di = trunc(p * d);
+    double lo = fma(dx, dy, -p);
+    double t = fma(-di, dm, p);
+    return t + lo;
+}
+
+// A per-lane "any of these held" flag kept as a wave mask in scalar registers: one vector compare
+// per update, the accumulation on the scalar unit.
+struct LaneAny {
+    unsigned long long m = 0;
+    GRNS_HD void set_if(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        m |= __ballot(p);
+#else
+        m |= p ? 1ull : 0ull;
+#endif
+    }
+    GRNS_HD bool any() const {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return (m >> (threadIdx.x & 63)) & 1ull;
+#else
+        return m != 0;
+#endif
+    }
+};
+
 GRNS_HD int mulmod_recip(int x, int y, int m, double d) {
     double q = ((double)x * (double)y) * d;
     int i = (int)q;

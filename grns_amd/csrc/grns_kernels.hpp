@@ -48,6 +48,7 @@ struct EvalLeaf {
     const int *xr;
     // the trees ask for leaves in ascending order: the constants come in views of 4 moduli
     cdouble *dw = nullptr, *dm = nullptr, *dinv = nullptr, *rd = nullptr, *ru = nullptr;
+    double smax = 0.0;  // max |s_i| before the int32 wrap: >= 2^31 leaves the number to the rare pass
     template <int I>
     GRNS_HD void get(double &lo, double &up) {
         if constexpr (I % 4 == 0) {
@@ -59,7 +60,8 @@ struct EvalLeaf {
             ru = const_view(cc->ru + I);
         }
         constexpr int k = I % 4;
-        double s = mulmod_recip_d((double)xr[I], dw[k], dm[k], dinv[k]);
+        double s = mulmod_recip_d_nowrap((double)xr[I], dw[k], dm[k], dinv[k]);
+        smax = fmax(smax, fabs(s));
         lo = dmul_rd(s, rd[k]);
         up = dmul_ru(s, ru[k]);
     }
@@ -84,7 +86,7 @@ GRNS_HD bool eval_hot(const DevCtx *c, const int (&xr)[N], interval_t &out) {
         out.low.frac = 0; out.low.exp = 0; out.upp.frac = 0; out.upp.exp = 0;
         return true;
     }
-    return (FAST || whl == whu) && (sumu >= c->accuracy);
+    return leaf.smax < 2147483648.0 && (FAST || whl == whu) && (sumu >= c->accuracy);
 }
 
 GRNS_D void store_interval(interval_t *p, const interval_t &v) {
@@ -270,13 +272,12 @@ __global__ __launch_bounds__(BLOCK) void k_eval_group(const DevCtx *__restrict__
 // is added with one rounding of at most 2^-44, klo_i is within 2^-74 of w_i/m_i - khi_i).
 GRNS_HD double filter_term(double dx, double kh, double kl) { return fract_d(fma(dx, kl, fract_d(dx * kh))); }
 
-// The flags of one number: residues canonical (x_i | (m_i - 1 - x_i) >= 0 for all i) and every
-// term in (2^-20, 1 - 2^-20).
+// The flags of one number: residues canonical and every term in (2^-20, 1 - 2^-20).
 struct FilterAcc {
     double acc = 0.0, lo = 1.0, hi = 0.0;
-    int bad = 0;
+    LaneAny bad;
     GRNS_HD void add(int xi, int m, double kh, double kl) {
-        bad |= xi | (m - 1 - xi);
+        bad.set_if((unsigned)xi >= (unsigned)m);
         const double f = filter_term((double)xi, kh, kl);
         lo = fmin(lo, f);
         hi = fmax(hi, f);
@@ -284,7 +285,7 @@ struct FilterAcc {
     }
     GRNS_HD double frac(const DevCtx *c, bool &regular, bool &canon, bool &terms) const {
         const double F = fract_d(acc);
-        canon = bad >= 0;
+        canon = !bad.any();
         terms = lo > 0x1p-20 && hi < 1.0 - 0x1p-20;
         regular = canon && terms && F > c->tau && F < 1.0 - c->tau;
         return F;
@@ -347,16 +348,21 @@ GRNS_HD int small_diff_sign(const DevCtx *c, const int (&xr)[N], const int (&yr)
     int t = xr[0] - yr[0];  // (-m0, m0)
     if (t > m0 / 2) t -= m0;
     else if (t <= -((m0 + 1) / 2)) t += m0;
-    bool ok = true;
+    LaneAny bad;
+    bad.set_if(2 * (long long)(t < 0 ? -t : t) >= (long long)c->m_min);
 #pragma unroll
-    for (int i = 1; i < N; i++) {
-        int m = c->m[i];
-        int d = xr[i] - yr[i];
-        d += (d < 0) ? m : 0;
-        int tm = t + ((t < 0) ? m : 0);  // t mod m_i, valid for |t| < m_i
-        ok &= (d == tm) && (2 * (long long)(t < 0 ? -t : t) < (long long)m);
+    for (int i0 = 0; i0 < N; i0 += 8) {
+        const DevCtx *cc = reload_point(c);
+        cint *cm = const_view(cc->m + i0);
+#pragma unroll
+        for (int k = (i0 == 0 ? 1 : 0); k < 8 && i0 + k < N; k++) {
+            // x_i - y_i - t is in (-3m/2, 3m/2): a multiple of m_i exactly when it is -m_i, 0 or m_i
+            const int u = xr[i0 + k] - yr[i0 + k] - t;
+            const int m = cm[k];
+            bad.set_if(u != 0 && u != m && u != -m);
+        }
     }
-    if (!ok) return 2;
+    if (bad.any()) return 2;
     return (t > 0) - (t < 0);
 }
 
@@ -477,10 +483,7 @@ GRNS_HD int cmp_lane(const DevCtx *c, const int (&xr)[N], const int (&yr)[N], co
     double e2 = 2.0 * c->err;
     if (fx - fy > e2) return 1;
     if (fy - fx > e2) return -1;
-    bool eq = true;
-#pragma unroll
-    for (int i = 0; i < N; i++) eq &= (xr[i] == yr[i]);
-    if (eq) return 0;
+    // equal values give t = 0 and every difference 0: the small-difference certificate returns 0
     int s = small_diff_sign<N>(c, xr, yr);
     if (!NEAR) return s;
     return s != 2 ? s : near_sign<2>(c, px, sx, py, sy);
@@ -1199,6 +1202,9 @@ GRNS_D int group_scale2pow(const DevCtx *c, const Group &g, int xi, unsigned D) 
         t -= 1;
     }
     while (t > 0) {
+        // a step maps residues that are all 0 to all 0 (c = 0, rank 0, remainder 0): once there, the
+        // remaining steps of the reference change nothing, which bounds the loop for any D
+        if (group_all(g, !act || result == 0)) break;
         int ci = act ? mulmod_recip(result, w, mi, dinv) : 0;
         k = group_rank_fast(c, g, ci);
         result = group_scaling_step(c, g, k, THRESH, result, ci);

@@ -121,6 +121,8 @@ grns_status build_constants(grns_ctx *c, const int *moduli, int n) {
     h.tau = tau;
     h.err = 0x1p-30;
     h.mr_lim = std::nextafter(1.0 / (double)moduli[n - 1], 2.0);
+    h.m_min = moduli[0];
+    for (int i = 1; i < n; i++) h.m_min = moduli[i] < h.m_min ? moduli[i] : h.m_min;
     h.big_moduli = 1;
     for (int i = 0; i < n; i++) h.big_moduli &= (moduli[i] >= (1 << 22));
     // RNS_EVAL_UNIT / INV_UNIT: M rounded down to 53 bits, then 1/Md and 1 - 1/Md rounded up and

@@ -219,3 +219,20 @@ def test_rare_queue_per_stream(ctxs, dev):
     torch.cuda.synchronize()
     for t in outs:
         assert (t.cpu().numpy() == ref).all()
+
+
+@pytest.mark.parametrize("n", [8, 32])
+def test_scale2pow_large_exponents(ctxs, dev, n):
+    """Exponents far above log2 M: the steps stop once the number is 0 (the reference's further
+    steps map 0 to 0), so the result is 0 and the call takes O(log2 M) steps for any D."""
+    c, o, mod = ctxs[n]
+    rnd = random.Random(90 + n)
+    vals = eval_inputs(mod, 200, seed=91 + n)
+    L = o.log2M
+    D_mid = np.array([rnd.randrange(L + 1, L + 3000) for _ in vals], np.uint32)
+    x = to_res(vals, mod)
+    y = c.scale2pow(_dev(x, dev), _dev(D_mid.view(np.int32), dev))
+    assert (y.cpu().numpy() == o.scale2pow_array(x, D_mid)).all()
+    D_huge = np.array([rnd.choice([1 << 31, 0xFFFFFFFF, rnd.randrange(1 << 20, 1 << 32)]) for _ in vals], np.uint32)
+    y = c.scale2pow(_dev(x, dev), _dev(D_huge.view(np.int32), dev))
+    assert (y.cpu().numpy() == 0).all()

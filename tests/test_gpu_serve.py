@@ -9,13 +9,7 @@ import pytest
 
 from tests.helpers import SETS, cmp_pairs, div_inputs, eval_inputs, to_res
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="the first GPU run of test_serve_alone_equals_mixed stopped producing output "
-                            "(killed after 180 s of silence); the dispatcher passes the same request shapes "
-                            "in host simulation (tests/test_serve_sim.py), the GPU-side cause is not found "
-                            "yet, so the GPU serving path stays off the GPU until it is"),
-]
+pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
@@ -36,10 +30,12 @@ class Req:
     def __init__(self, kind, ctx, o, mod, rnd, seed):
         import grns_amd as G
         self.kind, self.ctx = kind, ctx
+        self.keep = []  # the request's device inputs: the server reads them until the request completes
         cnt = rnd.choice([1, 17, 256, 1000, 3000])
         if kind == "evalcmp":
             xs, ys = cmp_pairs(mod, cnt, seed)
             x, y = _d(to_res(xs, mod)), _d(to_res(ys, mod))
+            self.keep += [x, y]
             ev = torch.zeros((cnt, 4), dtype=torch.float64, device="cuda")
             sg = torch.zeros(cnt, dtype=torch.int32, device="cuda")
             self.tasks = [G.make_task("eval", cnt, a=x, out0=ev), G.make_task("cmp", cnt, a=x, b=y, out0=sg)]
@@ -48,6 +44,7 @@ class Req:
         elif kind == "max":
             vals = eval_inputs(mod, cnt, seed)
             x = _d(to_res(vals, mod))
+            self.keep.append(x)
             r = torch.zeros(len(mod), dtype=torch.int32, device="cuda")
             i = torch.zeros(1, dtype=torch.int64, device="cuda")
             self.tasks = [G.make_task("max", cnt, a=x, out0=r, out1=i)]
@@ -57,6 +54,7 @@ class Req:
             k = max(1, cnt // {4: 1, 8: 20, 32: 100, 64: 300}[len(mod)])
             xs, ds = div_inputs(mod, k, seed)
             x, d = _d(to_res(xs, mod)), _d(to_res(ds, mod))
+            self.keep += [x, d]
             q, r = torch.zeros_like(x), torch.zeros_like(x)
             self.tasks = [G.make_task("div_fast", x.shape[1], a=x, b=d, out0=q, out1=r)]
             self.outs = [q, r]
@@ -65,6 +63,7 @@ class Req:
             vals = eval_inputs(mod, cnt, seed)
             x = _d(to_res(vals, mod))
             D = _d(np.array([rnd.randrange(0, 80) for _ in vals], np.int32))
+            self.keep += [x, D]
             y = torch.zeros_like(x)
             self.tasks = [G.make_task("scale2pow", cnt, a=x, out0=y, D=D)]
             self.outs = [y]
@@ -77,6 +76,7 @@ class Req:
             e = np.zeros((cnt, 2), dtype=ev["low"].dtype)
             e[:, 0], e[:, 1] = ev["low"], ev["upp"]
             X = (_d(dg), _d(sg), _d(e.view(np.float64).reshape(cnt, 2, 2)))
+            self.keep += list(X)
             Z = (torch.zeros_like(X[0]), torch.zeros_like(X[1]), torch.zeros_like(X[2]))
             self.tasks = [G.make_task("mpint_mul", cnt, mx=X, my=X, mz=Z)]
             self.outs = list(Z)
