@@ -16,7 +16,7 @@ the CPU restatement.
 Timing: W untimed warm-up steps, then K steps bracketed by torch.cuda.synchronize() and HIP
 events recorded on the stream the kernels are launched on (torch's current stream).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only max|eval|cmp]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only max,eval,cmp,batch,serve]
 """
 import argparse
 import json
@@ -53,21 +53,120 @@ def timed(fn, steps, warmup):
     return e0.elapsed_time(e1) / steps  # ms per step
 
 
-def roofline(bytes_per_step, ms):
-    gbs = bytes_per_step / (ms * 1e-3) / 1e9
+def kernel_ms(ctx, op, fn, steps):
+    """Average duration of op's streaming kernel over `steps` calls: HIP events recorded by the
+    library on the launch stream around that kernel alone."""
+    ctx.stage_timing(True)
+    for _ in range(steps):
+        fn()
+    ms, launches = ctx.stage_read(op)
+    ctx.stage_timing(False)
+    return ms / max(1, launches)
+
+
+def roofline(bytes_per_launch, ms, kernel):
+    """HBM roofline of one kernel: algorithmic bytes per launch over its average launch time."""
+    gbs = bytes_per_launch / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": load_traffic(kernel), "kernel": kernel,
+            "kernel_us": round(ms * 1e3, 2), "bytes_per_launch": bytes_per_launch}
 
 
 def load_traffic(name):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE), recorded in
-    profiles/traffic.json by the profiling run; null when absent."""
+    """HBM bytes per launch of kernel `name` measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    each scaled by the same counter's reading for a plain copy of known size), recorded in
+    profiles/traffic.json by `bench.py --traffic`; null when absent."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
-            return json.load(f).get(name)
-    except (OSError, ValueError):
+            k = json.load(f)["kernels"].get(name)
+        return None if k is None else k["traffic_bytes"]
+    except (OSError, ValueError, KeyError):
         return None
+
+
+COPY_BYTES = 1 << 31
+
+
+def pmc_workload():
+    """The program the counter runs profile: a plain 2 GiB device copy (calibration of FETCH_SIZE
+    and WRITE_SIZE), then two launches each of arg-max, evaluation and comparison on the headline
+    workload (n = 32, N = 2^24)."""
+    import torch
+
+    import grns_amd as G
+    from grns_amd.workload import Synth
+    mod = G.largest_primes_below(N_MODULI)
+    ctx, syn = G.Context(mod, 0), Synth(mod)
+    src = torch.zeros(COPY_BYTES // 4, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    torch.cuda.empty_cache()
+    x, _, _ = syn.max_planted(COUNT, seed=2)
+    ws = torch.empty(ctx.max_workspace_size(COUNT), dtype=torch.uint8, device="cuda")
+    out = torch.empty(N_MODULI, dtype=torch.int32, device="cuda")
+    oi = torch.empty(1, dtype=torch.int64, device="cuda")
+    for _ in range(2):
+        ctx.max(x, workspace=ws, out=out, out_idx=oi)
+    torch.cuda.synchronize()
+    del x
+    torch.cuda.empty_cache()
+    xe, ye, _ = syn.eval_cmp(COUNT, seed=1)
+    ev = torch.empty((COUNT, 4), dtype=torch.float64, device="cuda")
+    sg = torch.empty(COUNT, dtype=torch.int32, device="cuda")
+    for _ in range(2):
+        ctx.eval(xe, out=ev)
+        ctx.cmp(xe, ye, out=sg)
+    torch.cuda.synchronize()
+    log("[bench] pmc workload done")
+
+
+def traffic_from_counters(fetch_dir, write_dir, out_path):
+    """profiles/traffic.json from the counter_collection CSVs of a FETCH_SIZE run and a WRITE_SIZE
+    run of --pmc-workload: per kernel, the average KiB per dispatch times 1024 times the factor that
+    maps the counter's reading for the 2 GiB copy onto the bytes the copy moves."""
+    import csv
+    import glob
+
+    def short(name):
+        name = name[5:] if name.startswith("void ") else name
+        name = name.split("(")[0] if not name.startswith("(") else name
+        return name.replace("grns::", "")
+
+    def read(d, counter):
+        (path,) = glob.glob(os.path.join(d, "*counter_collection.csv"))
+        per = {}
+        copies = []
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            v = float(r["Counter_Value"])
+            if r["Kernel_Name"].startswith("__amd_rocclr_copyBuffer"):
+                copies.append((int(r["Grid_Size"]), v))
+            per.setdefault(short(r["Kernel_Name"]), []).append(v)
+        big = max(g for g, _ in copies)
+        copy_kib = sum(v for g, v in copies if g == big) / sum(1 for g, _ in copies if g == big)
+        return {k: sum(v) / len(v) for k, v in per.items()}, copy_kib
+
+    fetch, fcopy = read(fetch_dir, "FETCH_SIZE")
+    write, wcopy = read(write_dir, "WRITE_SIZE")
+    fs, ws_ = COPY_BYTES / (fcopy * 1024), COPY_BYTES / (wcopy * 1024)
+    kernels = {}
+    for k in sorted(fetch):
+        if not k.startswith("k_") or k not in write:
+            continue
+        fb, wb = fetch[k] * 1024 * fs, write[k] * 1024 * ws_
+        kernels[k] = {"fetch_kib": round(fetch[k], 3), "write_kib": round(write[k], 3), "fetch_bytes": round(fb),
+                      "write_bytes": round(wb), "traffic_bytes": round(fb + wb)}
+    res = {"calibration": {"copy_bytes": COPY_BYTES, "fetch_kib": round(fcopy, 3), "write_kib": round(wcopy, 3),
+                           "fetch_scale": round(fs, 4), "write_scale": round(ws_, 4)},
+           "kernels": kernels}
+    with open(out_path, "w") as f:
+        json.dump(res, f, indent=1)
+    log(f"[bench] wrote {out_path}")
 
 
 def batched_mpint_leg(steps, warmup, vectors=4096, max_len=65536, seed=4):
@@ -130,9 +229,166 @@ def batched_mpint_leg(steps, warmup, vectors=4096, max_len=65536, seed=4):
                        "add/sub/mul per vector",
            "elements": total, "batched_ms": round(ms_b, 4), "per_call_ms": round(ms_p, 4),
            "speedup": round(ms_p / ms_b, 2), "elem_ops_per_s": round(total / (ms_b * 1e-3)),
-           "roofline": roofline(bytes_per * total, ms_b), "tuned": cfg, "bit_exact_vs_per_call": same}
+           "roofline": roofline(bytes_per * total, ms_b, "k_batch_thread"), "tuned": cfg, "bit_exact_vs_per_call": same}
     batch.close()
     return out
+
+
+def serving_requests(n_req, seed):
+    """The serving workload's requests: each a dict with its context, batch tasks, a per-call
+    closure, served and per-call outputs, and its inputs (kept alive until the run ends)."""
+    import numpy as np
+    import torch
+
+    import grns_amd as G
+    from grns_amd.workload import Synth
+
+    rng = np.random.default_rng(seed)
+    sets = {k: G.largest_primes_below(k) for k in (8, 32, 64)}
+    ctx = {k: G.Context(m, 0) for k, m in sets.items()}
+    syn = {k: Synth(m) for k, m in sets.items()}
+    kinds = rng.choice(["evalcmp", "max", "div", "scale", "mpint"], size=n_req, p=[0.3, 0.2, 0.2, 0.1, 0.2])
+    sizes = np.floor(np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 22), size=n_req))).astype(np.int64)
+    which = rng.choice([8, 32, 64], size=n_req)
+    reqs = []
+    elements = 0
+    for r, (kind, cnt, k) in enumerate(zip(kinds, sizes, which)):
+        cnt = int(cnt)
+        c, s, n = ctx[k], syn[k], int(k)
+        s_seed = seed * 1000 + r
+        if kind == "evalcmp":
+            x, y, _ = s.eval_cmp(cnt, s_seed)
+            outs = lambda: (torch.empty((cnt, 4), dtype=torch.float64, device="cuda"),
+                            torch.empty(cnt, dtype=torch.int32, device="cuda"))
+            o_s, o_p = outs(), outs()
+            tasks = [G.make_task("eval", cnt, a=x, out0=o_s[0]), G.make_task("cmp", cnt, a=x, b=y, out0=o_s[1])]
+
+            def plain(c=c, x=x, y=y, o=o_p):
+                c.eval(x, out=o[0])
+                c.cmp(x, y, out=o[1])
+            keep = (x, y)
+        elif kind == "max":
+            x, _, _ = s.max_planted(cnt, s_seed)
+            o_s = (torch.empty(n, dtype=torch.int32, device="cuda"), torch.empty(1, dtype=torch.int64, device="cuda"))
+            o_p = (torch.empty(n, dtype=torch.int32, device="cuda"), torch.empty(1, dtype=torch.int64, device="cuda"))
+            tasks = [G.make_task("max", cnt, a=x, out0=o_s[0], out1=o_s[1])]
+            ws = torch.empty(c.max_workspace_size(cnt), dtype=torch.uint8, device="cuda")
+
+            def plain(c=c, x=x, o=o_p, ws=ws):
+                c.max(x, workspace=ws, out=o[0], out_idx=o[1])
+            keep = (x,)
+        elif kind == "div":
+            L = G.product(sets[k]).bit_length() - 1
+            x, d, _, _ = s.div(cnt, seed=s_seed, max_bits=L - 1, max_shift=L)
+            o_s = (torch.empty_like(x), torch.empty_like(x))
+            o_p = (torch.empty_like(x), torch.empty_like(x))
+            tasks = [G.make_task("div_fast", cnt, a=x, b=d, out0=o_s[0], out1=o_s[1])]
+
+            def plain(c=c, x=x, d=d, o=o_p):
+                c.div(x, d, fast=True, q=o[0], r=o[1])
+            keep = (x, d)
+        elif kind == "scale":
+            x = s.uniform(cnt, s_seed, 8)
+            D = s.uniform_u32(cnt, 1, 1900, s_seed, 9)
+            o_s, o_p = (torch.empty_like(x),), (torch.empty_like(x),)
+            tasks = [G.make_task("scale2pow", cnt, a=x, out0=o_s[0], D=D)]
+
+            def plain(c=c, x=x, D=D, o=o_p):
+                c.scale2pow(x, D, y=o[0])
+            keep = (x, D)
+        else:
+            L = G.product(sets[k]).bit_length() - 1
+            bits = torch.full((cnt,), min(480, (L - 4) // 2), dtype=torch.int32, device="cuda")
+
+            def operand(st):
+                dg = s.bits(bits, s_seed, st, top_set=False)
+                sg = s.uniform_u32(cnt, 0, 1, s_seed, st + 100)
+                return dg, sg, c.eval(dg).view(cnt, 2, 2)
+            X, Y = operand(10), operand(20)
+            o_s = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
+            o_p = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
+            tasks = [G.make_task("mpint_mul", cnt, mx=X, my=Y, mz=o_s)]
+
+            def plain(c=c, X=X, Y=Y, o=o_p, cnt=cnt):
+                c.mpint("mul", X, Y, o, cnt)
+            keep = (X, Y)
+        elements += cnt
+        reqs.append({"ctx": c, "tasks": tasks, "plain": plain, "out_s": o_s, "out_p": o_p, "keep": keep,
+                     "kind": kind, "n": n, "count": cnt})
+    torch.cuda.synchronize()
+    return reqs, elements, ctx
+
+
+def serving_leg(n_req=120, seed=7, budget=8 << 30):
+    """Serving stream: requests with Poisson arrivals over three resident moduli
+    sets (8, 32, 64 largest primes below 2^31), op mix 30% eval+cmp, 20% arg-max, 20% div_fast,
+    10% scale2pow, 20% mpint mul, sizes log-uniform in [2^10, 2^22], operands drawn like the
+    single-operation workloads (edge values in eval+cmp, planted maximum and near-ties in arg-max,
+    divisors of uniform bit length, exponents in [1, 1900], mpint magnitudes of up to 480 bits
+    within the range).  The same requests also run one after another through the plain per-call
+    entry points; every served output is compared with the per-call one byte for byte."""
+    import numpy as np
+    import torch
+
+    import grns_amd as G
+
+    reqs, elements, _ctx = serving_requests(n_req, seed)
+    rng = np.random.default_rng(seed + 1)
+
+    # plain per-call path: one request after another on the current stream
+    for q in reqs[:4]:
+        q["plain"]()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for q in reqs:
+        q["plain"]()
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+
+    def served(rate):
+        """Submit every request at Poisson arrival times of the given rate; wall time and stats."""
+        for q in reqs:
+            for o in q["out_s"]:
+                o.zero_()
+        torch.cuda.synchronize()
+        S = G.Server(0, hbm_budget=budget, slots=4, max_batch=16)
+        gaps = rng.exponential(1.0 / rate, size=n_req)
+        tickets = []
+        t0 = time.perf_counter()
+        t_next = t0
+        for q, g in zip(reqs, gaps):
+            t_next += g
+            while time.perf_counter() < t_next:
+                pass
+            tickets.append(S.submit(q["ctx"], q["tasks"]))
+        for t in tickets:
+            S.wait(t)
+        wall = time.perf_counter() - t0
+        st = S.stats()
+        S.close()
+        for q in reqs:
+            for a, b in zip(q["out_s"], q["out_p"]):
+                if not torch.equal(a.view(torch.uint8), b.view(torch.uint8)):
+                    raise SystemExit("served results differ from the per-call results")
+        return wall, st
+
+    cap = n_req / t_plain  # requests per second the per-call path sustains
+    served(4 * cap)  # warm-up of the server's kernels and graphs
+    t_sat, st_sat = served(16 * cap)  # offered load far above capacity: throughput
+    lam = 0.5 * cap
+    t_lat, st_lat = served(lam)  # offered load half the per-call capacity: latency
+    return {"workload": f"{n_req} requests, Poisson arrivals, mix 30% eval+cmp / 20% max / 20% div_fast / "
+                        f"10% scale2pow / 20% mpint mul, sizes log-uniform [2^10, 2^22], moduli sets of 8/32/64 "
+                        f"primes, {elements} elements",
+            "elements": elements, "per_call_s": round(t_plain, 4), "per_call_elem_per_s": round(elements / t_plain),
+            "saturated": {"offered_req_per_s": round(16 * cap, 1), "served_s": round(t_sat, 4),
+                          "served_elem_per_s": round(elements / t_sat), "speedup": round(t_plain / t_sat, 2),
+                          "p50_ms": round(st_sat["p50_ms"], 3), "p99_ms": round(st_sat["p99_ms"], 3),
+                          "launches": st_sat["launches"]},
+            "half_load": {"offered_req_per_s": round(lam, 1), "served_s": round(t_lat, 4),
+                          "p50_ms": round(st_lat["p50_ms"], 3), "p99_ms": round(st_lat["p99_ms"], 3),
+                          "launches": st_lat["launches"]},
+            "bit_exact_vs_per_call": True}
 
 
 def main():
@@ -143,12 +399,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="")
     ap.add_argument("--count", type=int, default=COUNT)
+    ap.add_argument("--pmc-workload", action="store_true", help="run the counter-profiling program and exit")
+    ap.add_argument("--traffic", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT"),
+                    help="write the traffic summary from two rocprofv3 --pmc runs of --pmc-workload and exit")
     args = ap.parse_args()
+    if args.traffic:
+        traffic_from_counters(*args.traffic)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the RNS path shards "
               "element ranges across GPUs with no exchange except the arg-max winners; see DESIGN.md).",
               file=sys.stderr)
         sys.exit(2)
+    if args.pmc_workload:
+        pmc_workload()
+        return
 
     import numpy as np
     import torch
@@ -179,7 +444,8 @@ def main():
         raise SystemExit(f"arg-max check failed: index {got_idx}, planted {p}")
     ms_max = timed(lambda: ctx.max(x, workspace=ws, out=out, out_idx=oidx), args.steps, args.warmup)
     max_bytes = 4 * n * N
-    log(f"[bench] max: {ms_max:.4f} ms/step")
+    kms_max = kernel_ms(ctx, "max", lambda: ctx.max(x, workspace=ws, out=out, out_idx=oidx), args.steps)
+    log(f"[bench] max: {ms_max:.4f} ms/step, stage-1 kernel {kms_max:.4f} ms")
 
     # ------------------------------------------------------------------ eval / cmp
     if only & {"eval", "cmp"}:
@@ -199,10 +465,12 @@ def main():
             if ev[si].cpu().numpy().tobytes() != ref.tobytes():
                 raise SystemExit("eval sample mismatch against the CPU restatement")
             ms_eval = timed(lambda: ctx.eval(xe, out=ev), args.steps, args.warmup)
+            kms = kernel_ms(ctx, "eval", lambda: ctx.eval(xe, out=ev), args.steps)
             eb = (4 * n + 32) * N
             extras["eval"] = {"ops_per_s": round(N / (ms_eval * 1e-3)), "ms_per_step": round(ms_eval, 4),
-                              "roofline": roofline(eb, ms_eval), "bytes_per_number": 4 * n + 32}
-            log(f"[bench] eval: {ms_eval:.4f} ms/step")
+                              "roofline": roofline(eb, kms, "k_eval<32, false, 1>"), "bytes_per_number": 4 * n + 32,
+                              "op_roofline_frac": round(eb / (ms_eval * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            log(f"[bench] eval: {ms_eval:.4f} ms/step, streaming kernel {kms:.4f} ms")
             del ev
         if "cmp" in only:
             sg = torch.empty(N, dtype=torch.int32, device="cuda")
@@ -211,10 +479,12 @@ def main():
             if not (sg[si].cpu().numpy() == ref).all():
                 raise SystemExit("cmp sample mismatch against the CPU restatement")
             ms_cmp = timed(lambda: ctx.cmp(xe, ye, out=sg), args.steps, args.warmup)
+            kms = kernel_ms(ctx, "cmp", lambda: ctx.cmp(xe, ye, out=sg), args.steps)
             cb = (8 * n + 4) * N
             extras["cmp"] = {"ops_per_s": round(N / (ms_cmp * 1e-3)), "ms_per_step": round(ms_cmp, 4),
-                             "roofline": roofline(cb, ms_cmp), "bytes_per_pair": 8 * n + 4}
-            log(f"[bench] cmp: {ms_cmp:.4f} ms/step")
+                             "roofline": roofline(cb, kms, "k_cmp<32, 1>"), "bytes_per_pair": 8 * n + 4,
+                             "op_roofline_frac": round(cb / (ms_cmp * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            log(f"[bench] cmp: {ms_cmp:.4f} ms/step, streaming kernel {kms:.4f} ms")
         status = ctx.status()
         if status:
             log(f"[bench] device status bits {status}")
@@ -225,6 +495,14 @@ def main():
         extras["batched"] = batched_mpint_leg(args.steps, args.warmup)
         log(f"[bench] batched mpint: {extras['batched']['batched_ms']} ms vs per-call "
             f"{extras['batched']['per_call_ms']} ms")
+        torch.cuda.empty_cache()
+
+    if "serve" in only or not args.only:
+        extras["serving"] = serving_leg()
+        sv = extras["serving"]
+        log(f"[bench] serving: saturated {sv['saturated']['served_s']} s vs per-call {sv['per_call_s']} s "
+            f"({sv['saturated']['speedup']}x); half load p50 {sv['half_load']['p50_ms']} ms, "
+            f"p99 {sv['half_load']['p99_ms']} ms")
         torch.cuda.empty_cache()
 
     # ------------------------------------------------------------------ CPU baseline
@@ -250,8 +528,8 @@ def main():
         log(f"[bench] cpu baseline {cpu['value']} ops/s on {threads} threads")
 
     value = N / (ms_max * 1e-3)
-    rl = roofline(max_bytes, ms_max)
-    rl["traffic"] = load_traffic("k_max_partial")
+    rl = roofline(max_bytes, kms_max, "k_max_partial<32, 1>")
+    rl["op_frac"] = round(max_bytes / (ms_max * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     line = {
         "metric": METRIC,
         "value": round(value),

@@ -77,7 +77,7 @@ DEBUG_EXPORTS = [
     "grns_debug_eval_full", "grns_debug_cmp_full",
     "grns_debug_mulmod_recip_d",
     "grns_debug_mulmod_recip", "grns_debug_addmod_recip", "grns_debug_bn_err", "grns_debug_fast_scalbn",
-    "grns_debug_filter_tau", "grns_debug_filter_err",
+    "grns_debug_filter_tau", "grns_debug_filter_err", "grns_debug_stage_timing", "grns_debug_stage_read",
 ]
 
 
@@ -137,6 +137,8 @@ def lib():
     L.grns_debug_filter_tau.argtypes = [P]
     L.grns_debug_filter_err.restype = D
     L.grns_debug_filter_err.argtypes = [P]
+    L.grns_debug_stage_timing.argtypes = [P, I]
+    L.grns_debug_stage_read.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     _lib = L
     return L
 
@@ -302,6 +304,17 @@ class Context:
         xa, ya, za = mpint_array(*x), mpint_array(*y), mpint_array(*z)
         _check(fn(self._h, C.byref(xa), C.byref(ya), C.byref(za), count,
                   _tptr(overflow) if overflow is not None else None, _stream_ptr(stream)), f"grns_mpint_{op}")
+
+    def stage_timing(self, enable=True):
+        """Time the streaming kernel of eval / cmp / max with HIP events on the launch stream."""
+        _check(lib().grns_debug_stage_timing(self._h, int(enable)), "grns_debug_stage_timing")
+
+    def stage_read(self, op):
+        """(summed kernel ms, launches) of op 'eval' / 'cmp' / 'max' since the last read."""
+        ms, k = C.c_double(), C.c_longlong()
+        _check(lib().grns_debug_stage_read(self._h, {"eval": 0, "cmp": 1, "max": 2}[op], C.byref(ms), C.byref(k)),
+               "grns_debug_stage_read")
+        return ms.value, k.value
 
     # ----------------------------------------------------------------- host test hooks
     def debug_eval_hot(self, residues, fast=False):

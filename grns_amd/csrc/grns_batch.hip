@@ -146,8 +146,14 @@ __global__ __launch_bounds__(BLOCK) void k_batch_group(const DevCtx *__restrict_
                     int xi = act ? T.a[oa] : 0;
                     int di = act ? T.b[oa] : 0;
                     int q, r;
-                    if (T.op == GRNS_OP_DIV) group_div<false>(c, g, xi, di, q, r, st);
-                    else group_div<true>(c, g, xi, di, q, r, st);
+                    if (g.G >= c->words && (g.G == 32 || g.G == 64) && div_words_ok(c, g, xi, di)) {
+                        if (g.G == 64) group_div_words<64>(c, g, xi, di, q, r);
+                        else group_div_words<32>(c, g, xi, di, q, r);
+                    } else if (T.op == GRNS_OP_DIV) {
+                        group_div<false>(c, g, xi, di, q, r, st);
+                    } else {
+                        group_div<true>(c, g, xi, di, q, r, st);
+                    }
                     if (act) {
                         reinterpret_cast<int *>(T.o0)[oa] = q;
                         reinterpret_cast<int *>(T.o1)[oa] = r;

@@ -126,6 +126,8 @@ struct DevCtx {
     int pow2_inv[THRESH][MAXN];              // RNS_POW2_INVERSE
     const int *pow2;                         // RNS_POW2, (log2M+1) x n, device memory
     const int *mrc_inv;                      // MRC_MULT_INV, n x n, device memory
+    int words;                               // 32-bit words of sum_i s_i (M / m_i) < n M
+    const unsigned *crt_words;               // (n + 1) x words: M / m_i for i < n, then M
 };
 
 // ---------------------------------------------------------------- BoundNear1 (src/dinterval.cuh)
@@ -646,6 +648,206 @@ GRNS_D int group_mrs_cmp(const Group &g, int a, int b) {
     int k = 63 - __clzll(diff);
     int ak = bcast_i(a, k), bk = bcast_i(b, k);
     return ak > bk ? 1 : -1;
+}
+
+// ---------------------------------------------------------------- numbers as 32-bit words
+// Division and scaling by powers of two run on the binary form of the numbers: word j (bits
+// 32j .. 32j+31) of a value sits in lane j of the group.  Word arithmetic is modulo 2^(32 GD); every
+// value handled stays below 2^(32 GD - 1) in magnitude, so the top bit of word GD-1 is the sign.
+// Carries between words are resolved with ballots: a word generates a carry, or passes an incoming
+// one on (all ones for an addition, zero for a subtraction), and the carries into all words follow
+// from one 64-bit addition of those bit masks.
+
+// The group's bits of a ballot, group lane 0 at bit 0.
+template <int GD>
+GRNS_D unsigned long long gballot(const Group &g, bool p) {
+    const unsigned long long b = __ballot(p);
+    if constexpr (GD == 64) return b;
+    else return (b >> g.base) & ((1ull << GD) - 1);
+}
+// Bit k: carry into word k, from the words that generate one (gen) or pass one on (prop);
+// gen and prop are disjoint.
+GRNS_D unsigned long long carry_in_bits(unsigned long long gen, unsigned long long prop) {
+    const unsigned long long a = gen | prop;
+    return (a + gen) ^ a ^ gen;
+}
+// The word of the next lower lane (0 into group lane 0), by a DPP wave shift.
+template <int GD>
+GRNS_D unsigned wlower(const Group &g, unsigned v) {
+    const unsigned r = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);  // wave_shr:1
+    return g.gl == 0 ? 0u : r;
+}
+// The word of the next higher lane (0 into the top lane of the group).
+template <int GD>
+GRNS_D unsigned whigher(const Group &g, unsigned v) {
+    const unsigned r = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);  // wave_shl:1
+    return g.gl == GD - 1 ? 0u : r;
+}
+// Word idx (uniform in the group) of the group's value, on every lane of the group.
+template <int GD>
+GRNS_D unsigned gword(const Group &g, unsigned v, int idx) {
+    if constexpr (GD == 64) return (unsigned)__builtin_amdgcn_readlane((int)v, idx);
+    else return (unsigned)__shfl((int)v, g.base + idx, 64);
+}
+// The value shifted up by k >= 0 words (words below k become 0).
+template <int GD>
+GRNS_D unsigned wshift_up(const Group &g, unsigned v, int k) {
+    const unsigned r = (unsigned)__shfl((int)v, g.base + (g.gl >= k ? g.gl - k : 0), 64);
+    return g.gl >= k ? r : 0u;
+}
+// Number of significant words (0 for zero).
+template <int GD>
+GRNS_D int wlen(const Group &g, unsigned v) {
+    const unsigned long long b = gballot<GD>(g, v != 0);
+    return b ? 64 - __clzll((long long)b) : 0;
+}
+template <int GD>
+GRNS_D bool wneg(const Group &g, unsigned v) { return (gword<GD>(g, v, GD - 1) >> 31) != 0; }
+
+// u += v
+template <int GD>
+GRNS_D void wadd(const Group &g, unsigned &u, unsigned v) {
+    const unsigned long long t = (unsigned long long)u + v;
+    const unsigned w = (unsigned)t;
+    const unsigned long long cin = carry_in_bits(gballot<GD>(g, (t >> 32) != 0), gballot<GD>(g, w == 0xffffffffu));
+    u = w + (unsigned)((cin >> g.gl) & 1);
+}
+// u -= q v (q < 2^32, the same on every lane of the group)
+template <int GD>
+GRNS_D void wmul_sub(const Group &g, unsigned &u, unsigned v, unsigned q) {
+    const unsigned long long p = (unsigned long long)v * q;
+    const unsigned hp = wlower<GD>(g, (unsigned)(p >> 32));
+    const long long t = (long long)u - (long long)(unsigned)p - (long long)hp;  // in (-2^33, 2^32)
+    const unsigned w = (unsigned)t;
+    const unsigned bp = wlower<GD>(g, (unsigned)(-(t >> 32)));  // borrow of 0, 1 or 2 from below
+    const unsigned w2 = w - bp;
+    const unsigned long long bin = carry_in_bits(gballot<GD>(g, w < bp), gballot<GD>(g, w2 == 0));
+    u = w2 - (unsigned)((bin >> g.gl) & 1);
+}
+
+// The value of canonical residues (residue xi on lane gl < n) as words, by the Chinese remainder
+// theorem: x = sum_i s_i (M / m_i) - k M with s_i = x_i w_i mod m_i and k = floor(sum_i s_i / m_i).
+// k comes from a double sum and is checked against the words when that sum is near an integer.
+template <int GD>
+GRNS_D unsigned group_words(const DevCtx *c, const Group &g, int xi) {
+    const bool act = g.gl < g.L;
+    const int s = act ? mulmod_recip(xi, c->w[g.gl], c->m[g.gl], c->dinv[g.gl]) : 0;
+    const double sum = group_sum_d(g, act ? (double)s * c->dinv[g.gl] : 0.0);
+    const double kf = floor(sum);
+    const double fr = sum - kf;
+    const int WW = c->words;
+    const bool wl = g.gl < WW;
+    const unsigned *tw = c->crt_words + g.gl;
+    unsigned long long lo = 0, hi = 0;
+    for (int i = 0; i < g.L; i++) {
+        const unsigned si = (unsigned)bcast_i(s, g.base + i);
+        const unsigned long long p = (unsigned long long)si * (wl ? tw[(size_t)i * WW] : 0u);
+        lo += (unsigned)p;
+        hi += p >> 32;
+    }
+    // words of sum_i s_i M_i: lo_j + hi_{j-1} (< 2^39), then the carries
+    const unsigned hlo = wlower<GD>(g, (unsigned)hi), hhi = wlower<GD>(g, (unsigned)(hi >> 32));
+    const unsigned long long v = lo + hlo + ((unsigned long long)hhi << 32);
+    const unsigned cp = wlower<GD>(g, (unsigned)(v >> 32));
+    const unsigned long long t = (unsigned long long)(unsigned)v + cp;
+    const unsigned w = (unsigned)t;
+    const unsigned long long cin = carry_in_bits(gballot<GD>(g, (t >> 32) != 0), gballot<GD>(g, w == 0xffffffffu));
+    unsigned X = w + (unsigned)((cin >> g.gl) & 1);
+    const unsigned mw = wl ? tw[(size_t)g.L * WW] : 0u;
+    wmul_sub<GD>(g, X, mw, (unsigned)kf);
+    if (!(fr > 0x1p-30 && fr < 1.0 - 0x1p-30)) {
+        if (wneg<GD>(g, X)) {
+            wadd<GD>(g, X, mw);
+        } else {
+            unsigned Y = X;
+            wmul_sub<GD>(g, Y, mw, 1u);
+            if (!wneg<GD>(g, Y)) X = Y;
+        }
+    }
+    return X;
+}
+
+// Residue modulo m_gl (lanes gl < n) of the value held in the first nw words.
+template <int GD>
+GRNS_D int group_words_residue(const DevCtx *c, const Group &g, unsigned v, int nw) {
+    const bool act = g.gl < g.L;
+    const int n = c->n;
+    const int *p2 = c->pow2 + (act ? g.gl : 0);
+    unsigned long long lo = 0, hi = 0;
+    if (nw > (c->log2M >> 5) + 1) nw = (c->log2M >> 5) + 1;  // a value below M has no more words
+    for (int k = 0; k < nw; k++) {
+        const unsigned wk = gword<GD>(g, v, k);
+        const unsigned long long p = (unsigned long long)wk * (unsigned)(act ? p2[(size_t)(32 * k) * n] : 0);
+        lo += (unsigned)p;
+        hi += p >> 32;
+    }
+    if (!act) return 0;
+    const int m = c->m[g.gl];
+    const double dinv = c->dinv[g.gl];
+    const long long rl = rem_trunc((long long)lo, m, dinv), rh = rem_trunc((long long)hi, m, dinv);
+    return (int)rem_trunc(rh * (long long)c->c32[g.gl] + rl, m, dinv);
+}
+
+// x = q d + r for canonical residues with d != 0: long division of the words (TAOCP vol. 2,
+// 4.3.1, Algorithm D), the quotient's residues from its words, r = x - q d in the residues.
+// Quotient and remainder of two integers are unique, so these are the residues rns_div and
+// rns_div_fast (src/rnsdiv.cuh:19-114) return for such inputs.  Needs GD >= c->words.
+template <int GD>
+GRNS_D void group_div_words(const DevCtx *c, const Group &g, int xi, int di, int &q, int &r) {
+    const bool act = g.gl < g.L;
+    unsigned X = group_words<GD>(c, g, xi);
+    unsigned D = group_words<GD>(c, g, di);
+    const int wx = wlen<GD>(g, X), wd = wlen<GD>(g, D);
+    q = 0;
+    r = xi;
+    if (wx < wd) return;
+    const int sh = __clz((int)gword<GD>(g, D, wd - 1));
+    if (sh) {
+        D = (D << sh) | (wlower<GD>(g, D) >> (32 - sh));
+        X = (X << sh) | (wlower<GD>(g, X) >> (32 - sh));
+    }
+    const unsigned long long v1 = gword<GD>(g, D, wd - 1);
+    const unsigned long long v2 = wd >= 2 ? gword<GD>(g, D, wd - 2) : 0u;
+    const int top = wx - wd;
+    unsigned Ds = wshift_up<GD>(g, D, top);
+    unsigned Q = 0;
+    for (int j = top; j >= 0; j--) {
+        const unsigned long long u2 = gword<GD>(g, X, j + wd), u1 = gword<GD>(g, X, j + wd - 1);
+        const unsigned long long u0 = wd >= 2 ? gword<GD>(g, X, j + wd - 2) : 0u;
+        const unsigned long long num = (u2 << 32) | u1;
+        unsigned long long qh, rh;
+        if (u2 >= v1) {
+            qh = 0xffffffffull;
+            rh = num - qh * v1;
+        } else {
+            qh = (unsigned long long)((double)num / (double)v1);
+            if (qh > 0xffffffffull) qh = 0xffffffffull;
+            long long rr = (long long)(num - qh * v1);
+            while (rr < 0) { qh--; rr += (long long)v1; }
+            while (rr >= (long long)v1) { qh++; rr -= (long long)v1; }
+            rh = (unsigned long long)rr;
+        }
+        while (rh < (1ull << 32) && qh * v2 > ((rh << 32) | u0)) {
+            qh--;
+            rh += v1;
+        }
+        if (qh) {
+            wmul_sub<GD>(g, X, Ds, (unsigned)qh);
+            if (wneg<GD>(g, X)) {
+                qh--;
+                wadd<GD>(g, X, Ds);
+            }
+        }
+        if (g.gl == j) Q = (unsigned)qh;
+        Ds = whigher<GD>(g, Ds);
+    }
+    const int qr = group_words_residue<GD>(c, g, Q, top + 1);
+    if (act) {
+        const int m = c->m[g.gl];
+        const double dinv = c->dinv[g.gl];
+        q = qr;
+        r = psubmod(xi, mulmod3(qr, di, m, dinv), m, dinv);
+    }
 }
 
 GRNS_D er_t bcast_er(er_t v, int lane) {

@@ -16,6 +16,7 @@ struct grns_ctx {
     int *d_pow2 = nullptr;
     int *d_mrc = nullptr;
     int *d_status = nullptr;
+    unsigned *d_crt = nullptr;      // DevCtx::crt_words
     int max_blocks = 2048;         // arg-max stage-1 grid: one resident wave of blocks on the device
     // rare-item queues of the per-call eval / cmp kernels, one per stream (made on first use)
     struct RareQueue {
@@ -25,8 +26,21 @@ struct grns_ctx {
     };
     std::mutex qmu;
     std::vector<RareQueue> queues;
+    // optional timing of the first (streaming) kernel of eval / cmp / max: HIP events recorded on
+    // the caller's stream around that kernel while enabled; pairs are reused round-robin and folded
+    // into the total before reuse
+    struct StageTimer {
+        std::vector<void *> ev;    // 2 * slots events (start, end)
+        std::vector<char> pending;  // slot holds an unread pair
+        int next = 0;
+        double ms = 0.0;
+        long long launches = 0;
+    };
+    bool timing = false;
+    StageTimer timers[3];
     std::vector<int> pow2;
     std::vector<int> mrc;
+    std::vector<unsigned> crt;
 };
 
 namespace grns {

@@ -1120,11 +1120,49 @@ GRNS_D void group_div(const DevCtx *c, const Group &g, int xi, int di, int &q, i
     }
 }
 
+// Whether the pair can go through the word division: canonical residues and d != 0 (the word
+// form also needs a group of at least c->words lanes).
+GRNS_D bool div_words_ok(const DevCtx *c, const Group &g, int xi, int di) {
+    const bool act = g.gl < g.L;
+    const unsigned m = act ? (unsigned)c->m[g.gl] : 1u;
+    return group_all(g, !act || ((unsigned)xi < m && (unsigned)di < m)) && group_any(g, act && di != 0);
+}
+
+// Division of the pairs div_words_ok accepts (group of GD lanes per pair); k_div takes the others.
+template <int GD>
+__global__ __launch_bounds__(BLOCK) void k_div_words(const DevCtx *__restrict__ c, const int *__restrict__ x,
+                                                     const int *__restrict__ d, long long smod, long long snum,
+                                                     long long count, int *__restrict__ q, int *__restrict__ r) {
+    const Group g = make_group(GD, c->n);
+    constexpr int P = 64 / GD;
+    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
+    for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
+        const long long j = j0 + (threadIdx.x & 63) / GD;
+        if (j < count) {
+            const bool act = g.gl < g.L;
+            const long long off = j * snum + (long long)g.gl * smod;
+            const int xi = act ? x[off] : 0;
+            const int di = act ? d[off] : 0;
+            if (div_words_ok(c, g, xi, di)) {
+                int qq, rr;
+                group_div_words<GD>(c, g, xi, di, qq, rr);
+                if (act) {
+                    q[off] = qq;
+                    r[off] = rr;
+                }
+            }
+        }
+    }
+}
+
+// skip_words: leave the pairs that k_div_words has divided.
 template <bool FAST>
 __global__ __launch_bounds__(BLOCK) void k_div(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
                                                const int *__restrict__ d, long long smod, long long snum,
                                                long long count, int *__restrict__ q, int *__restrict__ r,
-                                               const long long *__restrict__ order, int *__restrict__ status) {
+                                               const long long *__restrict__ order, int *__restrict__ status,
+                                               int skip_words) {
     Group g = make_group(G, c->n);
     const int P = 64 / G;
     long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
@@ -1136,6 +1174,7 @@ __global__ __launch_bounds__(BLOCK) void k_div(const DevCtx *__restrict__ c, int
             long long off = j * snum + (long long)g.gl * smod;
             int xi = g.gl < g.L ? x[off] : 0;
             int di = g.gl < g.L ? d[off] : 0;
+            if (skip_words && div_words_ok(c, g, xi, di)) continue;
             int qq, rr, st = 0;
             group_div<FAST>(c, g, xi, di, qq, rr, st);
             if (g.gl < g.L) {

@@ -166,7 +166,33 @@ grns_status build_constants(grns_ctx *c, const int *moduli, int n) {
     c->mrc.assign((size_t)n * n, 0);
     for (int i = 0; i < n; i++)
         for (int j = i + 1; j < n; j++) c->mrc[(size_t)i * n + j] = inverse_mod(moduli[i], moduli[j]);
+    // M / m_i and M as 32-bit words, one word more than M has (room for sum_i s_i M / m_i < n M)
+    const int WW = (int)limbs.size() + 1;
+    h.words = WW;
+    c->crt.assign((size_t)(n + 1) * WW, 0u);
+    for (int i = 0; i < n; i++) {
+        std::vector<unsigned> mi(1, 1u);
+        for (int j = 0; j < n; j++) {
+            if (j == i) continue;
+            unsigned long long carry = 0;
+            for (auto &l : mi) {
+                unsigned long long t = (unsigned long long)l * (unsigned)moduli[j] + carry;
+                l = (unsigned)t;
+                carry = t >> 32;
+            }
+            if (carry) mi.push_back((unsigned)carry);
+        }
+        for (size_t k = 0; k < mi.size(); k++) c->crt[(size_t)i * WW + k] = mi[k];
+    }
+    for (size_t k = 0; k < limbs.size(); k++) c->crt[(size_t)n * WW + k] = limbs[k];
     return GRNS_OK;
+}
+
+// Lanes per number of the word division: a power of two with room for every word (0: none fits).
+int words_group(const grns_ctx *c) {
+    int G = groups_for(c->n);
+    while (G < c->h.words) G <<= 1;
+    return G <= 64 ? G : 0;
 }
 
 
@@ -258,6 +284,38 @@ int max_grid_for(const grns_ctx *c) {
     return cus * per;
 }
 
+// Timing of the streaming kernel of eval (op 0), cmp (op 1) and max (op 2): an event pair around
+// each launch while timing is on.  A slot that still holds an unread pair is folded into the total
+// before it is reused.  Not thread-safe: one caller times one context.
+constexpr int TIMER_SLOTS = 64;
+
+void fold_slot(grns_ctx::StageTimer &t, int k) {
+    float ms = 0.f;
+    if (!t.pending[k]) return;
+    if (hipEventSynchronize((hipEvent_t)t.ev[2 * k + 1]) == hipSuccess &&
+        hipEventElapsedTime(&ms, (hipEvent_t)t.ev[2 * k], (hipEvent_t)t.ev[2 * k + 1]) == hipSuccess) {
+        t.ms += ms;
+        t.launches++;
+    }
+    t.pending[k] = 0;
+}
+
+int stage_begin(grns_ctx *c, int op, void *stream) {
+    if (!c->timing) return -1;
+    auto &t = c->timers[op];
+    int k = t.next;
+    t.next = (k + 1) % TIMER_SLOTS;
+    fold_slot(t, k);
+    if (hipEventRecord((hipEvent_t)t.ev[2 * k], S(stream)) != hipSuccess) return -1;
+    return k;
+}
+
+void stage_end(grns_ctx *c, int op, int k, void *stream) {
+    if (k < 0) return;
+    auto &t = c->timers[op];
+    if (hipEventRecord((hipEvent_t)t.ev[2 * k + 1], S(stream)) == hipSuccess) t.pending[k] = 1;
+}
+
 }  // namespace
 
 extern "C" {
@@ -280,6 +338,7 @@ grns_status grns_ctx_create(const int *moduli, int n, int device, grns_ctx **out
     }
     DeviceGuard guard(device);
     if (hipMalloc(&c->d, sizeof(DevCtx)) != hipSuccess ||
+        hipMalloc(&c->d_crt, sizeof(unsigned) * c->crt.size()) != hipSuccess ||
         hipMalloc(&c->d_pow2, sizeof(int) * c->pow2.size()) != hipSuccess ||
         hipMalloc(&c->d_mrc, sizeof(int) * c->mrc.size()) != hipSuccess ||
         hipMalloc(&c->d_status, sizeof(int)) != hipSuccess) {
@@ -288,7 +347,9 @@ grns_status grns_ctx_create(const int *moduli, int n, int device, grns_ctx **out
     }
     c->h.pow2 = c->d_pow2;
     c->h.mrc_inv = c->d_mrc;
-    if (hipMemcpy(c->d_pow2, c->pow2.data(), sizeof(int) * c->pow2.size(), hipMemcpyHostToDevice) != hipSuccess ||
+    c->h.crt_words = c->d_crt;
+    if (hipMemcpy(c->d_crt, c->crt.data(), sizeof(unsigned) * c->crt.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_pow2, c->pow2.data(), sizeof(int) * c->pow2.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d_mrc, c->mrc.data(), sizeof(int) * c->mrc.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d, &c->h, sizeof(DevCtx), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->d_status, 0, sizeof(int)) != hipSuccess) {
@@ -306,12 +367,15 @@ grns_status grns_ctx_destroy(grns_ctx *c) {
         DeviceGuard guard(c->device);
         if (c->d) (void)hipFree(c->d);
         if (c->d_pow2) (void)hipFree(c->d_pow2);
+        if (c->d_crt) (void)hipFree(c->d_crt);
         if (c->d_mrc) (void)hipFree(c->d_mrc);
         if (c->d_status) (void)hipFree(c->d_status);
         for (auto &q : c->queues) {
             (void)hipFree(q.items);
             (void)hipFree(q.ctr);
         }
+        for (auto &t : c->timers)
+            for (void *e : t.ev) (void)hipEventDestroy((hipEvent_t)e);
     }
     delete c;
     return GRNS_OK;
@@ -373,6 +437,7 @@ grns_status grns_eval(grns_ctx *c, const int *x, long long smod, long long snum,
         grns_status qs = rare_queue(c, stream, q);                                                               \
         if (qs != GRNS_OK) return qs;                                                                            \
         dim3 rg((unsigned)rare_grid(count));                                                                     \
+        int tk = stage_begin(c, 0, stream);                                                                      \
         if (mm) {                                                                                                \
             if (fast) hipLaunchKernelGGL((k_eval<NN, true, LD_MM>), grid, blk, 0, S(stream), c->d, x, smod, snum,  \
                                          cnt, o, q);                                                             \
@@ -384,6 +449,7 @@ grns_status grns_eval(grns_ctx *c, const int *x, long long smod, long long snum,
             else hipLaunchKernelGGL((k_eval<NN, false, LD_ANY>), grid, blk, 0, S(stream), c->d, x, smod, snum,     \
                                     cnt, o, q);                                                                  \
         }                                                                                                        \
+        stage_end(c, 0, tk, stream);                                                                             \
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                                \
         if (fast) hipLaunchKernelGGL((k_eval_rare<NN, true>), rg, blk, 0, S(stream), c->d, x, smod, snum, cnt, o,  \
                                      c->d_status, q);                                                            \
@@ -426,10 +492,12 @@ grns_status grns_cmp(grns_ctx *c, const int *x, long long xsm, long long xsn, co
         grns_status qs = rare_queue(c, stream, q);                                                                 \
         if (qs != GRNS_OK) return qs;                                                                              \
         dim3 rg((unsigned)rare_grid(count));                                                                       \
+        int tk = stage_begin(c, 1, stream);                                                                        \
         if (mm) hipLaunchKernelGGL((k_cmp<NN, LD_MM>), grid, blk, 0, S(stream), c->d, x, xsm, xsn, y, ysm, ysn, cnt,  \
                                    sign, q);                                                                       \
         else hipLaunchKernelGGL((k_cmp<NN, LD_ANY>), grid, blk, 0, S(stream), c->d, x, xsm, xsn, y, ysm, ysn, cnt,     \
                                 sign, q);                                                                          \
+        stage_end(c, 1, tk, stream);                                                                               \
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                                  \
         hipLaunchKernelGGL((k_cmp_rare<NN>), rg, blk, 0, S(stream), c->d, x, xsm, xsn, y, ysm, ysn, cnt, sign,       \
                            c->d_status, q);                                                                        \
@@ -464,8 +532,23 @@ grns_status grns_div(grns_ctx *c, const int *x, const int *d, long long smod, lo
     int G = groups_for(c->n);
     dim3 grid((unsigned)grid_groups(count, G)), blk(BLOCK);
     long long cnt = (long long)count;
-    if (fast) hipLaunchKernelGGL((k_div<true>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, (const long long *)nullptr, c->d_status);
-    else hipLaunchKernelGGL((k_div<false>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, (const long long *)nullptr, c->d_status);
+    // canonical pairs with d != 0: word division; then the reference loop for what is left
+    const int GW = words_group(c);
+    if (GW) {
+        dim3 gw((unsigned)grid_groups(count, GW));
+        switch (GW) {
+            case 2: hipLaunchKernelGGL((k_div_words<2>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r); break;
+            case 4: hipLaunchKernelGGL((k_div_words<4>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r); break;
+            case 8: hipLaunchKernelGGL((k_div_words<8>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r); break;
+            case 16: hipLaunchKernelGGL((k_div_words<16>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r); break;
+            case 32: hipLaunchKernelGGL((k_div_words<32>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r); break;
+            default: hipLaunchKernelGGL((k_div_words<64>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r); break;
+        }
+        if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;
+    }
+    const int skip = GW ? 1 : 0;
+    if (fast) hipLaunchKernelGGL((k_div<true>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, (const long long *)nullptr, c->d_status, skip);
+    else hipLaunchKernelGGL((k_div<false>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, (const long long *)nullptr, c->d_status, skip);
     return launch_check();
 }
 
@@ -522,12 +605,15 @@ grns_status grns_max(grns_ctx *c, const int *x, long long smod, long long snum, 
     const bool mm = snum == 1 && count <= ((size_t)1 << 29);
     dim3 grid((unsigned)nb), blk(BLOCK), fblk(FBLOCK);
     long long cnt = (long long)count;
+    int tk;
 #define GRNS_MAX_CASE(NN)                                                                                    \
     case NN:                                                                                                 \
+        tk = stage_begin(c, 2, stream);                                                                      \
         if (mm) hipLaunchKernelGGL((k_max_partial<NN, LD_MM>), grid, blk, 0, S(stream), c->d, x, smod, snum, \
                                    cnt, ch, part, part_res);                                                 \
         else hipLaunchKernelGGL((k_max_partial<NN, LD_ANY>), grid, blk, 0, S(stream), c->d, x, smod, snum,    \
                                 cnt, ch, part, part_res);                                                    \
+        stage_end(c, 2, tk, stream);                                                                         \
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                            \
         hipLaunchKernelGGL((k_max_final<NN>), dim3(1), fblk, 0, S(stream), c->d, x, smod, snum, part,        \
                            part_res, (int)nb, out_residues, out_idx);                                        \
@@ -626,6 +712,7 @@ DevCtx host_view(const grns_ctx *c) {
     DevCtx h = c->h;
     h.pow2 = c->pow2.data();
     h.mrc_inv = c->mrc.data();
+    h.crt_words = c->crt.data();
     return h;
 }
 template <int N>
@@ -725,6 +812,40 @@ int grns_debug_cmp_lane(const grns_ctx *c, const int *x, const int *y) {
         default: return -2;
     }
 }
+grns_status grns_debug_stage_timing(grns_ctx *c, int enable) {
+    if (!c || c->device < 0) return GRNS_ERR_ARG;
+    DeviceGuard guard(c->device);
+    for (auto &t : c->timers) {
+        if (t.ev.empty()) {
+            t.ev.assign(2 * TIMER_SLOTS, nullptr);
+            t.pending.assign(TIMER_SLOTS, 0);
+            for (auto &e : t.ev) {
+                hipEvent_t h;
+                if (hipEventCreate(&h) != hipSuccess) return GRNS_ERR_HIP;
+                e = h;
+            }
+        }
+        for (int k = 0; k < TIMER_SLOTS; k++) fold_slot(t, k);
+        t.ms = 0.0;
+        t.launches = 0;
+        t.next = 0;
+    }
+    c->timing = enable != 0;
+    return GRNS_OK;
+}
+
+grns_status grns_debug_stage_read(grns_ctx *c, int op, double *ms_total, long long *launches) {
+    if (!c || c->device < 0 || op < 0 || op > 2 || !ms_total || !launches) return GRNS_ERR_ARG;
+    DeviceGuard guard(c->device);
+    auto &t = c->timers[op];
+    for (int k = 0; k < (int)t.pending.size(); k++) fold_slot(t, k);
+    *ms_total = t.ms;
+    *launches = t.launches;
+    t.ms = 0.0;
+    t.launches = 0;
+    return GRNS_OK;
+}
+
 int grns_debug_ceil_log2(double s) { return ceil_log2_libm(s); }
 double grns_debug_mulmod_recip_d(double x, double y, double m, double d) { return mulmod_recip_d(x, y, m, d); }
 int grns_debug_mulmod_recip(int x, int y, int m, double d) { return mulmod_recip(x, y, m, d); }

@@ -24,6 +24,12 @@ double grns_debug_bn_err(double c);
 double grns_debug_fast_scalbn(double x, int n);
 double grns_debug_filter_tau(const grns_ctx *ctx);
 double grns_debug_filter_err(const grns_ctx *ctx);
+/* Kernel timing: while enabled, grns_eval, grns_cmp and grns_max record HIP events on the caller's
+ * stream around their streaming kernel (op 0: the evaluation kernel, 1: the comparison kernel,
+ * 2: arg-max stage 1).  grns_debug_stage_read waits for the recorded launches and returns the summed
+ * kernel time in ms and the launch count since the last read, then clears them.  Not thread-safe. */
+grns_status grns_debug_stage_timing(grns_ctx *ctx, int enable);
+grns_status grns_debug_stage_read(grns_ctx *ctx, int op, double *ms_total, long long *launches);
 /* Scheduler test hook: a server whose launches run nothing and complete delay_us after issue;
  * it accepts host-only contexts (device -1), so admission and batching run without a GPU. */
 grns_status grns_serve_create_sim(const grns_serve_config *cfg, int delay_us, grns_server **out);

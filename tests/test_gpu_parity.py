@@ -92,6 +92,67 @@ def test_div_parity(ctxs, dev, n, fast):
     assert c.status() == 0
 
 
+def _div_cases(mod, count, seed):
+    """Dividends and divisors that exercise the word division: random sizes, divisors of one word,
+    all-ones and power-of-2^32 words, quotients near word boundaries, d > x, d == x, x == M - 1."""
+    from grns_amd import product
+    M = product(mod)
+    L = M.bit_length()
+    rnd = random.Random(seed)
+    xs, ds = [], []
+    for _ in range(count):
+        x = rnd.randrange(M)
+        d = rnd.getrandbits(rnd.randrange(1, L)) or 1
+        xs.append(x); ds.append(min(d, M - 1))
+    for _ in range(count // 4):  # x = q d + r with chosen r
+        d = rnd.randrange(1, M)
+        q = rnd.randrange(M // d + 1)
+        xs.append(min(q * d + rnd.choice([0, d - 1, rnd.randrange(d)]), M - 1)); ds.append(d)
+    for k in range(1, (L - 1) // 32 + 1):
+        for v in ((1 << (32 * k)) - 1, 1 << (32 * k), (1 << (32 * k)) + 1):
+            if v < M:
+                xs += [v, v, M - 1, v, M - 1]
+                ds += [1, (1 << 32) - 1, v, v, (1 << 31) + 1]
+    xs += [0, 5, 7, 7, M - 1, M - 1, M - 1, M // 2]
+    ds += [1, 7, 7, 1, 1, M - 1, 2, M // 2 + 1]
+    return xs, ds
+
+
+@pytest.mark.parametrize("n", [4, 8, 32, 64])
+@pytest.mark.parametrize("fast", [True, False])
+def test_div_exact(ctxs, dev, n, fast):
+    """Quotient and remainder residues against exact integer division (the residues the CPU path
+    returns for canonical inputs with d != 0 are those of the unique quotient and remainder)."""
+    c, o, mod = ctxs[n]
+    xs, ds = _div_cases(mod, 1500, seed=60 + n)
+    x, d = to_res(xs, mod), to_res(ds, mod)
+    q, r = c.div(_dev(x, dev), _dev(d, dev), fast=fast)
+    eq = to_res([a // b for a, b in zip(xs, ds)], mod)
+    er = to_res([a % b for a, b in zip(xs, ds)], mod)
+    qn, rn = q.cpu().numpy(), r.cpu().numpy()
+    bad = np.nonzero((qn != eq).any(axis=0) | (rn != er).any(axis=0))[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first x={xs[bad[0]]} d={ds[bad[0]]}"
+    assert c.status() == 0
+
+
+@pytest.mark.parametrize("n", [8, 32])
+def test_div_noncanonical_residues(ctxs, dev, n):
+    """Residues at or above their modulus take the reference loop; results equal the CPU path's."""
+    c, o, mod = ctxs[n]
+    xs, ds = div_inputs(mod, 40, seed=70 + n)
+    x, d = to_res(xs, mod), to_res(ds, mod)
+    rnd = random.Random(71)
+    for j in range(0, x.shape[1], 3):
+        i = rnd.randrange(len(mod))
+        tgt = x if j % 2 else d
+        if int(tgt[i, j]) + mod[i] < 2 ** 31:
+            tgt[i, j] += mod[i]
+    q, r = c.div(_dev(x, dev), _dev(d, dev), fast=True)
+    rq, rr = o.div_array(x, d, fast=True)
+    assert (q.cpu().numpy() == rq).all() and (r.cpu().numpy() == rr).all()
+    assert c.status() == 0
+
+
 @pytest.mark.parametrize("n", [8, 32, 64])
 def test_scale2pow_parity(ctxs, dev, n):
     c, o, mod = ctxs[n]
