@@ -78,6 +78,7 @@ DEBUG_EXPORTS = [
     "grns_debug_mulmod_recip_d",
     "grns_debug_mulmod_recip", "grns_debug_addmod_recip", "grns_debug_bn_err", "grns_debug_fast_scalbn",
     "grns_debug_filter_tau", "grns_debug_filter_err", "grns_debug_stage_timing", "grns_debug_stage_read",
+    "grns_debug_reference_loops",
 ]
 
 
@@ -138,6 +139,7 @@ def lib():
     L.grns_debug_filter_err.restype = D
     L.grns_debug_filter_err.argtypes = [P]
     L.grns_debug_stage_timing.argtypes = [P, I]
+    L.grns_debug_reference_loops.argtypes = [P, I]
     L.grns_debug_stage_read.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     _lib = L
     return L
@@ -304,6 +306,10 @@ class Context:
         xa, ya, za = mpint_array(*x), mpint_array(*y), mpint_array(*z)
         _check(fn(self._h, C.byref(xa), C.byref(ya), C.byref(za), count,
                   _tptr(overflow) if overflow is not None else None, _stream_ptr(stream)), f"grns_mpint_{op}")
+
+    def reference_loops(self, on=True):
+        """Division through the reference's loop for every pair (test hook)."""
+        _check(lib().grns_debug_reference_loops(self._h, int(on)), "grns_debug_reference_loops")
 
     def stage_timing(self, enable=True):
         """Time the streaming kernel of eval / cmp / max with HIP events on the launch stream."""

@@ -731,7 +731,9 @@ GRNS_D void wmul_sub(const Group &g, unsigned &u, unsigned v, unsigned q) {
 template <int GD>
 GRNS_D unsigned group_words(const DevCtx *c, const Group &g, int xi) {
     const bool act = g.gl < g.L;
-    const int s = act ? mulmod_recip(xi, c->w[g.gl], c->m[g.gl], c->dinv[g.gl]) : 0;
+    // exact reduction: the reciprocal one (mulmod_recip) can return s_i - m_i or wrap, and the
+    // unsigned word sums below need 0 <= s_i < m_i
+    const int s = act ? mulmod3(xi, c->w[g.gl], c->m[g.gl], c->dinv[g.gl]) : 0;
     const double sum = group_sum_d(g, act ? (double)s * c->dinv[g.gl] : 0.0);
     const double kf = floor(sum);
     const double fr = sum - kf;
@@ -841,11 +843,16 @@ GRNS_D void group_div_words(const DevCtx *c, const Group &g, int xi, int di, int
         if (g.gl == j) Q = (unsigned)qh;
         Ds = whigher<GD>(g, Ds);
     }
+    const bool qnz = gballot<GD>(g, Q != 0) != 0;
     const int qr = group_words_residue<GD>(c, g, Q, top + 1);
     if (act) {
         const int m = c->m[g.gl];
         const double dinv = c->dinv[g.gl];
-        q = qr;
+        // The reference builds q by reciprocal additions (mod_add, src/modular.cuh:29-34), and one
+        // whose operands sum to exactly m returns m itself when RN(m * RN(1/m)) < 1.  A nonzero
+        // quotient divisible by such an m_i is always reached through such an addition (every
+        // partial quotient and the final +1 are nonzero), so its residue there is m_i, not 0.
+        q = (qnz && qr == 0 && (double)m * dinv < 1.0) ? m : qr;
         r = psubmod(xi, mulmod3(qr, di, m, dinv), m, dinv);
     }
 }

@@ -533,7 +533,7 @@ grns_status grns_div(grns_ctx *c, const int *x, const int *d, long long smod, lo
     dim3 grid((unsigned)grid_groups(count, G)), blk(BLOCK);
     long long cnt = (long long)count;
     // canonical pairs with d != 0: word division; then the reference loop for what is left
-    const int GW = words_group(c);
+    const int GW = c->reference_loops ? 0 : words_group(c);
     if (GW) {
         dim3 gw((unsigned)grid_groups(count, GW));
         switch (GW) {
@@ -843,6 +843,12 @@ grns_status grns_debug_stage_read(grns_ctx *c, int op, double *ms_total, long lo
     *launches = t.launches;
     t.ms = 0.0;
     t.launches = 0;
+    return GRNS_OK;
+}
+
+grns_status grns_debug_reference_loops(grns_ctx *c, int on) {
+    if (!c) return GRNS_ERR_ARG;
+    c->reference_loops = on != 0;
     return GRNS_OK;
 }
 

@@ -115,23 +115,69 @@ def _div_cases(mod, count, seed):
                 ds += [1, (1 << 32) - 1, v, v, (1 << 31) + 1]
     xs += [0, 5, 7, 7, M - 1, M - 1, M - 1, M // 2]
     ds += [1, 7, 7, 1, 1, M - 1, 2, M // 2 + 1]
+    # quotients divisible by a modulus (the CPU path's reciprocal addition can return m_i there)
+    for m in mod:
+        for k in (1, 2, 12345):
+            for dd in (1, 3, (1 << 40) + 7, rnd.randrange(1, max(2, M // (k * m)))):
+                if k * m * dd < M:
+                    xs.append(k * m * dd + rnd.randrange(dd)); ds.append(dd)
     return xs, ds
 
 
 @pytest.mark.parametrize("n", [4, 8, 32, 64])
 @pytest.mark.parametrize("fast", [True, False])
 def test_div_exact(ctxs, dev, n, fast):
-    """Quotient and remainder residues against exact integer division (the residues the CPU path
-    returns for canonical inputs with d != 0 are those of the unique quotient and remainder)."""
+    """Quotient and remainder residues equal the CPU path's, and are congruent to those of exact
+    integer division."""
     c, o, mod = ctxs[n]
-    xs, ds = _div_cases(mod, 1500, seed=60 + n)
+    xs, ds = _div_cases(mod, 1500 if n < 64 else 600, seed=60 + n)
     x, d = to_res(xs, mod), to_res(ds, mod)
     q, r = c.div(_dev(x, dev), _dev(d, dev), fast=fast)
+    qn, rn = q.cpu().numpy(), r.cpu().numpy()
+    rq, rr = o.div_array(x, d, fast=fast)
+    bad = np.nonzero((qn != rq).any(axis=0) | (rn != rr).any(axis=0))[0]
+    assert bad.size == 0, f"{bad.size} mismatches with the CPU path, first x={xs[bad[0]]} d={ds[bad[0]]}"
+    m = np.array(mod, np.int64)[:, None]
     eq = to_res([a // b for a, b in zip(xs, ds)], mod)
     er = to_res([a % b for a, b in zip(xs, ds)], mod)
-    qn, rn = q.cpu().numpy(), r.cpu().numpy()
-    bad = np.nonzero((qn != eq).any(axis=0) | (rn != er).any(axis=0))[0]
-    assert bad.size == 0, f"{bad.size} mismatches, first x={xs[bad[0]]} d={ds[bad[0]]}"
+    assert ((qn.astype(np.int64) % m) == eq).all() and ((rn.astype(np.int64) % m) == er).all()
+    assert c.status() == 0
+
+
+def _crt(col, mod, M):
+    v = 0
+    for r, m in zip(col, mod):
+        Mi = M // m
+        v += int(r) * Mi * pow(Mi, -1, m)
+    return v % M
+
+
+@pytest.mark.parametrize("n,count,fast", [(8, 1 << 20, True), (32, 1 << 19, True), (64, 1 << 17, True),
+                                          (32, 1 << 16, False), (64, 1 << 14, False)])
+def test_div_words_against_reference_loop(ctxs, dev, n, count, fast):
+    """The word division against the reference's loop run on the device (same kernel code as the
+    CPU restatement), on the division workload's distribution: every pair agrees, except pairs on
+    which the loop itself is not the exact quotient (an int32 wrap in its reciprocal reduction)."""
+    import grns_amd as G
+    from grns_amd.workload import Synth
+    c, o, mod = ctxs[n]
+    M = G.product(mod)
+    L = M.bit_length() - 1
+    x, d, _, _ = Synth(mod).div(count, seed=500 + n, max_bits=min(1980, L - 1), max_shift=min(1900, L))
+    q1, r1, q2, r2 = (torch.empty_like(x) for _ in range(4))
+    c.div(x, d, fast=fast, q=q1, r=r1)
+    c.reference_loops(True)
+    try:
+        c.div(x, d, fast=fast, q=q2, r=r2)
+    finally:
+        c.reference_loops(False)
+    idx = torch.nonzero(((q1 != q2) | (r1 != r2)).any(dim=0)).flatten().cpu().numpy()
+    assert idx.size <= 3
+    for j in idx:
+        xv, dv = _crt(x[:, j].cpu().numpy(), mod, M), _crt(d[:, j].cpu().numpy(), mod, M)
+        exact = ([(xv // dv) % m for m in mod], [(xv % dv) % m for m in mod])
+        assert (q1[:, j].cpu().numpy().tolist(), r1[:, j].cpu().numpy().tolist()) == exact
+        assert (q2[:, j].cpu().numpy().tolist(), r2[:, j].cpu().numpy().tolist()) != exact
     assert c.status() == 0
 
 
