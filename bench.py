@@ -16,7 +16,7 @@ the CPU restatement.
 Timing: W untimed warm-up steps, then K steps bracketed by torch.cuda.synchronize() and HIP
 events recorded on the stream the kernels are launched on (torch's current stream).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only max,eval,cmp,batch,serve]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only max,eval,cmp,div,batch,serve]
 """
 import argparse
 import json
@@ -167,6 +167,46 @@ def traffic_from_counters(fetch_dir, write_dir, out_path):
     with open(out_path, "w") as f:
         json.dump(res, f, indent=1)
     log(f"[bench] wrote {out_path}")
+
+
+def div_scale_leg(steps, warmup, n_mod=64, count=1 << 20, seed=3):
+    """General division and power-of-two scaling: 64 largest primes below 2^31 (~1983-bit range),
+    2^20 pairs, dividends uniform in [0, M), divisors of uniform bit length in [1, 1980] with
+    uniform bits, exponents uniform in [1, 1900].  A sample of 256 results is checked against
+    exact integer arithmetic before timing."""
+    import numpy as np
+    import torch
+
+    import grns_amd as G
+    from grns_amd.workload import Synth
+    mod = G.largest_primes_below(n_mod)
+    M = G.product(mod)
+    ctx, syn = G.Context(mod, 0), Synth(mod)
+    x, d, D, _ = syn.div(count, seed=seed, max_bits=1980, max_shift=1900)
+    q, r, y = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
+    ctx.div(x, d, fast=True, q=q, r=r)
+    ctx.scale2pow(x, D, y=y)
+    torch.cuda.synchronize()
+    idx = np.random.default_rng(seed).choice(count, 256, replace=False)
+    xs, ds, qs, rs, ys = (t[:, idx].cpu().numpy() for t in (x, d, q, r, y))
+    Ds = D[idx].cpu().numpy().view(np.uint32)
+    inv = [pow(M // m, -1, m) for m in mod]
+
+    def val(col):
+        return sum(int(v) * (M // m) * w for v, m, w in zip(col, mod, inv)) % M
+    for j in range(len(idx)):
+        xv, dv = val(xs[:, j]), val(ds[:, j])
+        if (val(qs[:, j]), val(rs[:, j])) != (xv // dv, xv % dv) or val(ys[:, j]) != xv >> int(Ds[j]):
+            raise SystemExit("division / scaling sample differs from exact integer arithmetic")
+    ms_df = timed(lambda: ctx.div(x, d, fast=True, q=q, r=r), steps, warmup)
+    ms_d = timed(lambda: ctx.div(x, d, fast=False, q=q, r=r), max(1, steps // 4), 1)
+    ms_s = timed(lambda: ctx.scale2pow(x, D, y=y), steps, warmup)
+    return {"workload": f"n={n_mod} (largest primes below 2^31) N=2^{count.bit_length() - 1} pairs, divisor bits "
+                        "uniform in [1, 1980], exponents uniform in [1, 1900]",
+            "div_fast_ops_per_s": round(count / (ms_df * 1e-3)), "div_fast_ms": round(ms_df, 4),
+            "div_ops_per_s": round(count / (ms_d * 1e-3)), "div_ms": round(ms_d, 4),
+            "scale2pow_ops_per_s": round(count / (ms_s * 1e-3)), "scale2pow_ms": round(ms_s, 4),
+            "sample_checked_exact": 256}
 
 
 def batched_mpint_leg(steps, warmup, vectors=4096, max_len=65536, seed=4):
@@ -489,6 +529,12 @@ def main():
         if status:
             log(f"[bench] device status bits {status}")
         del xe, ye
+        torch.cuda.empty_cache()
+
+    if "div" in only or not args.only:
+        extras["div_scale"] = div_scale_leg(args.steps, args.warmup)
+        log(f"[bench] div_fast {extras['div_scale']['div_fast_ms']} ms, div {extras['div_scale']['div_ms']} ms, "
+            f"scale2pow {extras['div_scale']['scale2pow_ms']} ms per 2^20")
         torch.cuda.empty_cache()
 
     if "batch" in only or not args.only:

@@ -146,9 +146,14 @@ __global__ __launch_bounds__(BLOCK) void k_batch_group(const DevCtx *__restrict_
                     int xi = act ? T.a[oa] : 0;
                     int di = act ? T.b[oa] : 0;
                     int q, r;
-                    if (g.G >= c->words && (g.G == 32 || g.G == 64) && div_words_ok(c, g, xi, di)) {
-                        if (g.G == 64) group_div_words<64>(c, g, xi, di, q, r);
-                        else group_div_words<32>(c, g, xi, di, q, r);
+                    if (g.G >= c->words && g.G >= 4 && div_words_ok(c, g, xi, di)) {
+                        switch (g.G) {
+                            case 4: group_div_words<4>(c, g, xi, di, q, r); break;
+                            case 8: group_div_words<8>(c, g, xi, di, q, r); break;
+                            case 16: group_div_words<16>(c, g, xi, di, q, r); break;
+                            case 32: group_div_words<32>(c, g, xi, di, q, r); break;
+                            default: group_div_words<64>(c, g, xi, di, q, r); break;
+                        }
                     } else if (T.op == GRNS_OP_DIV) {
                         group_div<false>(c, g, xi, di, q, r, st);
                     } else {
@@ -257,7 +262,9 @@ grns_status enqueue_plan(grns_ctx *c, const TaskPlan &P, const BatchTask *d_task
         }
     }
     if (P.ngrp) {
-        hipLaunchKernelGGL(k_batch_group, grid, blk, 0, s, c->d, groups_for(c->n), d_tasks + P.nthr, P.ngrp,
+        // group width: the word division's when it has one (the group tasks are mostly divisions)
+        const int G = words_group(c) ? words_group(c) : groups_for(c->n);
+        hipLaunchKernelGGL(k_batch_group, grid, blk, 0, s, c->d, G, d_tasks + P.nthr, P.ngrp,
                            P.grp_tiles, cfg.tile, d_counters + 1, c->d_status);
     }
     return hipGetLastError() == hipSuccess ? GRNS_OK : GRNS_ERR_HIP;
@@ -531,15 +538,48 @@ struct grns_server {
 
 namespace {
 
+// Tasks at least this long run as their own streaming launches (the per-call kernels) on the
+// slot's stream; shorter ones are fused into one persistent launch.
+constexpr size_t DIRECT_MIN = (size_t)1 << 15;
+
+grns_status launch_direct(grns_ctx *c, const grns_batch_task &t, Slot &sl) {
+    switch (t.op) {
+        case GRNS_OP_EVAL:
+        case GRNS_OP_EVAL_FAST:
+            return grns_eval(c, t.a, t.a_mod_stride, t.a_num_stride, t.count, (grns_interval *)t.out0,
+                             t.op == GRNS_OP_EVAL_FAST, sl.stream);
+        case GRNS_OP_CMP:
+            return grns_cmp(c, t.a, t.a_mod_stride, t.a_num_stride, t.b, t.b_mod_stride, t.b_num_stride, t.count,
+                            (int *)t.out0, sl.stream);
+        case GRNS_OP_MPINT_ADD: return grns_mpint_add(c, &t.mx, &t.my, &t.mz, t.count, t.overflow, sl.stream);
+        case GRNS_OP_MPINT_SUB: return grns_mpint_sub(c, &t.mx, &t.my, &t.mz, t.count, t.overflow, sl.stream);
+        case GRNS_OP_MPINT_MUL: return grns_mpint_mul(c, &t.mx, &t.my, &t.mz, t.count, t.overflow, sl.stream);
+        case GRNS_OP_DIV:
+        case GRNS_OP_DIV_FAST:
+            return grns_div(c, t.a, t.b, t.a_mod_stride, t.a_num_stride, t.count, (int *)t.out0, (int *)t.out1,
+                            t.op == GRNS_OP_DIV_FAST, sl.stream);
+        case GRNS_OP_SCALE2POW:
+            return grns_scale2pow(c, t.a, t.a_mod_stride, t.a_num_stride, t.D, t.count, (int *)t.out0, sl.stream);
+        case GRNS_OP_MAX:
+            return grns_max(c, t.a, t.a_mod_stride, t.a_num_stride, t.count, (int *)t.out0, (long long *)t.out1,
+                            sl.max_ws, sl.max_ws_bytes, sl.stream);
+        default: return GRNS_ERR_ARG;
+    }
+}
+
 grns_status launch_group(grns_server *S, Slot &sl) {
     if (S->sim_delay_us >= 0) {  // host simulation: the launch "completes" after a fixed delay
         sl.sim_done = Clock::now() + std::chrono::microseconds(S->sim_delay_us);
         return GRNS_OK;
     }
-    std::vector<grns_batch_task> flat;
+    std::vector<grns_batch_task> flat, direct;
     grns_ctx *c = sl.reqs.front().ctx;
     for (auto &r : sl.reqs)
-        for (auto &t : r.tasks) flat.push_back(t);
+        for (auto &t : r.tasks) {
+            if (t.count == 0) continue;
+            if (t.op == GRNS_OP_MAX || t.count >= DIRECT_MIN) direct.push_back(t);
+            else flat.push_back(t);
+        }
     grns_batch_config bc{S->cfg.tile, S->cfg.blocks_per_cu, 1, 0};
     TaskPlan plan;
     make_plan(flat.data(), (int)flat.size(), c->n, bc, plan);
@@ -551,11 +591,9 @@ grns_status launch_group(grns_server *S, Slot &sl) {
             st = GRNS_ERR_HIP;
         if (st == GRNS_OK) st = enqueue_plan(c, plan, sl.d_tasks, sl.d_counters, bc, S->cus, sl.stream);
     }
-    for (auto &t : flat) {
+    for (auto &t : direct) {
         if (st != GRNS_OK) break;
-        if (t.op != GRNS_OP_MAX || t.count == 0) continue;
-        st = grns_max(c, t.a, t.a_mod_stride, t.a_num_stride, t.count, (int *)t.out0, (long long *)t.out1, sl.max_ws,
-                      sl.max_ws_bytes, sl.stream);
+        st = launch_direct(c, t, sl);
     }
     if (hipEventRecord(sl.done, sl.stream) != hipSuccess && st == GRNS_OK) st = GRNS_ERR_HIP;
     return st;

@@ -51,3 +51,14 @@ inline int groups_for(int n) {
     return G;
 }
 }  // namespace grns
+
+// Lanes per number of the word division: a power of two with room for every residue and every
+// word of the CRT sum (0: no group of 64 lanes has room).
+inline int words_group(const grns_ctx *c) {
+    int G = grns::groups_for(c->n);
+    while (G < c->h.words) G <<= 1;
+    return G <= 64 ? G : 0;
+}
+
+namespace grns {
+}  // namespace grns

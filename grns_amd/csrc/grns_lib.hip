@@ -188,12 +188,6 @@ grns_status build_constants(grns_ctx *c, const int *moduli, int n) {
     return GRNS_OK;
 }
 
-// Lanes per number of the word division: a power of two with room for every word (0: none fits).
-int words_group(const grns_ctx *c) {
-    int G = groups_for(c->n);
-    while (G < c->h.words) G <<= 1;
-    return G <= 64 ? G : 0;
-}
 
 
 hipStream_t S(void *s) { return (hipStream_t)s; }
