@@ -1128,23 +1128,32 @@ GRNS_D bool div_words_ok(const DevCtx *c, const Group &g, int xi, int di) {
     return group_all(g, !act || ((unsigned)xi < m && (unsigned)di < m)) && group_any(g, act && di != 0);
 }
 
-// Division of the pairs div_words_ok accepts (group of GD lanes per pair); k_div takes the others.
+// Marker of a division left to the rare pass after a queue overflow (written to q's first residue;
+// the word division never writes it).
+constexpr int DIV_MARK = (int)0x80000000u;
+
+// Division of the pairs div_words_ok accepts (group of GD lanes per pair); the others go to the
+// rare queue, and k_div_rare runs the reference loop for them.
 template <int GD>
 __global__ __launch_bounds__(BLOCK) void k_div_words(const DevCtx *__restrict__ c, const int *__restrict__ x,
                                                      const int *__restrict__ d, long long smod, long long snum,
-                                                     long long count, int *__restrict__ q, int *__restrict__ r) {
+                                                     long long count, int *__restrict__ q, int *__restrict__ r,
+                                                     RareQ rq) {
     const Group g = make_group(GD, c->n);
     constexpr int P = 64 / GD;
     const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
     const long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
     for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
         const long long j = j0 + (threadIdx.x & 63) / GD;
-        if (j < count) {
+        const bool valid = j < count;
+        bool ok = false;
+        if (valid) {
             const bool act = g.gl < g.L;
             const long long off = j * snum + (long long)g.gl * smod;
             const int xi = act ? x[off] : 0;
             const int di = act ? d[off] : 0;
-            if (div_words_ok(c, g, xi, di)) {
+            ok = div_words_ok(c, g, xi, di);
+            if (ok) {
                 int qq, rr;
                 group_div_words<GD>(c, g, xi, di, qq, rr);
                 if (act) {
@@ -1153,16 +1162,48 @@ __global__ __launch_bounds__(BLOCK) void k_div_words(const DevCtx *__restrict__ 
                 }
             }
         }
+        const bool rare = valid && !ok && g.gl == 0;
+        if (!rare_push(rq, rare, j)) q[j * snum] = DIV_MARK;
     }
 }
 
-// skip_words: leave the pairs that k_div_words has divided.
+// Rare pass of the division: the reference loop, one wave group per queued pair (and, after a
+// queue overflow, per marked pair).
+template <bool FAST>
+__global__ __launch_bounds__(BLOCK) void k_div_rare(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
+                                                    const int *__restrict__ d, long long smod, long long snum,
+                                                    long long count, int *__restrict__ q, int *__restrict__ r,
+                                                    int *__restrict__ status, RareQ rq) {
+    Group g = make_group(G, c->n);
+    const int P = 64 / G;
+    const long long gid = (((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6) * P + (threadIdx.x & 63) / G;
+    const long long ngroups = ((long long)gridDim.x * BLOCK >> 6) * P;
+    const unsigned pushed = rq.ctr[0];
+    const long long nq = pushed < rq.cap ? pushed : rq.cap;
+    const long long total = nq + (rq.ctr[2] ? count : 0);
+    int st = 0;
+    for (long long k = gid; k < total; k += ngroups) {
+        const long long j = rare_item(rq, nq, k, total);
+        if (k >= nq && q[j * snum] != DIV_MARK) continue;
+        const long long off = j * snum + (long long)g.gl * smod;
+        const int xi = g.gl < g.L ? x[off] : 0;
+        const int di = g.gl < g.L ? d[off] : 0;
+        int qq, rr;
+        group_div<FAST>(c, g, xi, di, qq, rr, st);
+        if (g.gl < g.L) {
+            q[off] = qq;
+            r[off] = rr;
+        }
+    }
+    if (g.gl == 0 && st) atomicOr(status, st);
+}
+
+// The reference loop for every pair.
 template <bool FAST>
 __global__ __launch_bounds__(BLOCK) void k_div(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
                                                const int *__restrict__ d, long long smod, long long snum,
                                                long long count, int *__restrict__ q, int *__restrict__ r,
-                                               const long long *__restrict__ order, int *__restrict__ status,
-                                               int skip_words) {
+                                               const long long *__restrict__ order, int *__restrict__ status) {
     Group g = make_group(G, c->n);
     const int P = 64 / G;
     long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
@@ -1174,7 +1215,6 @@ __global__ __launch_bounds__(BLOCK) void k_div(const DevCtx *__restrict__ c, int
             long long off = j * snum + (long long)g.gl * smod;
             int xi = g.gl < g.L ? x[off] : 0;
             int di = g.gl < g.L ? d[off] : 0;
-            if (skip_words && div_words_ok(c, g, xi, di)) continue;
             int qq, rr, st = 0;
             group_div<FAST>(c, g, xi, di, qq, rr, st);
             if (g.gl < g.L) {

@@ -526,23 +526,30 @@ grns_status grns_div(grns_ctx *c, const int *x, const int *d, long long smod, lo
     int G = groups_for(c->n);
     dim3 grid((unsigned)grid_groups(count, G)), blk(BLOCK);
     long long cnt = (long long)count;
-    // canonical pairs with d != 0: word division; then the reference loop for what is left
+    // canonical pairs with d != 0: word division; the reference loop for the others (rare pass)
     const int GW = c->reference_loops ? 0 : words_group(c);
     if (GW) {
-        dim3 gw((unsigned)grid_groups(count, GW));
+        RareQ rq;
+        grns_status qs = rare_queue(c, stream, rq);
+        if (qs != GRNS_OK) return qs;
+        dim3 gw((unsigned)grid_groups(count, GW)), rg((unsigned)rare_grid(count));
         switch (GW) {
-            case 2: hipLaunchKernelGGL((k_div_words<2>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r); break;
-            case 4: hipLaunchKernelGGL((k_div_words<4>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r); break;
-            case 8: hipLaunchKernelGGL((k_div_words<8>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r); break;
-            case 16: hipLaunchKernelGGL((k_div_words<16>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r); break;
-            case 32: hipLaunchKernelGGL((k_div_words<32>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r); break;
-            default: hipLaunchKernelGGL((k_div_words<64>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r); break;
+            case 2: hipLaunchKernelGGL((k_div_words<2>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r, rq); break;
+            case 4: hipLaunchKernelGGL((k_div_words<4>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r, rq); break;
+            case 8: hipLaunchKernelGGL((k_div_words<8>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r, rq); break;
+            case 16: hipLaunchKernelGGL((k_div_words<16>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r, rq); break;
+            case 32: hipLaunchKernelGGL((k_div_words<32>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r, rq); break;
+            default: hipLaunchKernelGGL((k_div_words<64>), gw, blk, 0, S(stream), c->d, x, d, smod, snum, cnt, q, r, rq); break;
         }
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;
+        if (fast) hipLaunchKernelGGL((k_div_rare<true>), rg, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, c->d_status, rq);
+        else hipLaunchKernelGGL((k_div_rare<false>), rg, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, c->d_status, rq);
+        if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;
+        hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), rq.ctr);
+        return launch_check();
     }
-    const int skip = GW ? 1 : 0;
-    if (fast) hipLaunchKernelGGL((k_div<true>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, (const long long *)nullptr, c->d_status, skip);
-    else hipLaunchKernelGGL((k_div<false>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, (const long long *)nullptr, c->d_status, skip);
+    if (fast) hipLaunchKernelGGL((k_div<true>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, (const long long *)nullptr, c->d_status);
+    else hipLaunchKernelGGL((k_div<false>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, (const long long *)nullptr, c->d_status);
     return launch_check();
 }
 

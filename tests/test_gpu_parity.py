@@ -181,6 +181,33 @@ def test_div_words_against_reference_loop(ctxs, dev, n, count, fast):
     assert c.status() == 0
 
 
+def test_div_rare_queue_overflow(ctxs, dev):
+    """More pairs for the reference loop than the rare queue holds (negative, non-canonical residues
+    in every other pair): the overflow is marked and the rare pass still divides every pair."""
+    import grns_amd as G
+    from grns_amd.workload import Synth
+    c, o, mod = ctxs[8]
+    count = (1 << 21) + 5000
+    L = G.product(mod).bit_length() - 1
+    x, d, _, _ = Synth(mod).div(count, seed=77, max_bits=L - 1, max_shift=L)
+    x[0, ::2] -= mod[0]
+    q1, r1, q2, r2 = (torch.empty_like(x) for _ in range(4))
+    c.div(x, d, fast=True, q=q1, r=r1)
+    c.reference_loops(True)
+    try:
+        c.div(x, d, fast=True, q=q2, r=r2)
+    finally:
+        c.reference_loops(False)
+    diff = ((q1 != q2) | (r1 != r2)).any(dim=0)
+    assert not bool(diff[::2].any())  # both ran the reference loop on the non-canonical pairs
+    M = G.product(mod)
+    for j in torch.nonzero(diff).flatten().cpu().numpy()[:4]:  # canonical pairs: words exact
+        xv, dv = _crt(x[:, j].cpu().numpy(), mod, M), _crt(d[:, j].cpu().numpy(), mod, M)
+        assert q1[:, j].cpu().numpy().tolist() == [(xv // dv) % m for m in mod]
+    assert int(diff.sum()) <= 4
+    assert c.status() == 0
+
+
 @pytest.mark.parametrize("n", [8, 32])
 def test_div_noncanonical_residues(ctxs, dev, n):
     """Residues at or above their modulus take the reference loop; results equal the CPU path's."""
