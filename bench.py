@@ -91,7 +91,7 @@ COPY_BYTES = 1 << 31
 def pmc_workload():
     """The program the counter runs profile: a plain 2 GiB device copy (calibration of FETCH_SIZE
     and WRITE_SIZE), then two launches each of arg-max, evaluation and comparison on the headline
-    workload (n = 32, N = 2^24)."""
+    workload (n = 32, N = 2^24), and of division and scaling on 2^18 pairs of the n = 64 workload."""
     import torch
 
     import grns_amd as G
@@ -120,6 +120,16 @@ def pmc_workload():
     for _ in range(2):
         ctx.eval(xe, out=ev)
         ctx.cmp(xe, ye, out=sg)
+    torch.cuda.synchronize()
+    del xe, ye, ev, sg
+    torch.cuda.empty_cache()
+    mod64 = G.largest_primes_below(64)
+    c64 = G.Context(mod64, 0)
+    x, d, D, _ = Synth(mod64).div(1 << 18, seed=3, max_bits=1980, max_shift=1900)
+    q, r = torch.empty_like(x), torch.empty_like(x)
+    for _ in range(2):
+        c64.div(x, d, fast=True, q=q, r=r)
+        c64.scale2pow(x, D, y=q)
     torch.cuda.synchronize()
     log("[bench] pmc workload done")
 
