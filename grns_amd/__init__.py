@@ -308,7 +308,7 @@ class Context:
                   _tptr(overflow) if overflow is not None else None, _stream_ptr(stream)), f"grns_mpint_{op}")
 
     def reference_loops(self, on=True):
-        """Division through the reference's loop for every pair (test hook)."""
+        """Division and scaling through the reference's loops for every number (test hook)."""
         _check(lib().grns_debug_reference_loops(self._h, int(on)), "grns_debug_reference_loops")
 
     def stage_timing(self, enable=True):

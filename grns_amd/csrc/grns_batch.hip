@@ -167,7 +167,19 @@ __global__ __launch_bounds__(BLOCK) void k_batch_group(const DevCtx *__restrict_
                 }
                 case GRNS_OP_SCALE2POW: {
                     int xi = act ? T.a[oa] : 0;
-                    int v = group_scale2pow(c, g, xi, T.D[j]);
+                    const unsigned Dj = T.D[j];
+                    bool ok = false;
+                    int v = 0;
+                    if (g.G >= c->words && g.G >= 4 && group_all(g, !act || (unsigned)xi < (unsigned)c->m[g.gl])) {
+                        switch (g.G) {
+                            case 4: v = group_scale2pow_words<4>(c, g, xi, Dj, ok); break;
+                            case 8: v = group_scale2pow_words<8>(c, g, xi, Dj, ok); break;
+                            case 16: v = group_scale2pow_words<16>(c, g, xi, Dj, ok); break;
+                            case 32: v = group_scale2pow_words<32>(c, g, xi, Dj, ok); break;
+                            default: v = group_scale2pow_words<64>(c, g, xi, Dj, ok); break;
+                        }
+                    }
+                    if (!ok) v = group_scale2pow(c, g, xi, Dj);
                     if (act) reinterpret_cast<int *>(T.o0)[oa] = v;
                     break;
                 }

@@ -857,6 +857,54 @@ GRNS_D void group_div_words(const DevCtx *c, const Group &g, int xi, int di, int
     }
 }
 
+// rns_scale2pow (src/rnsscal.cuh:93-119) replayed on the words of a canonical x.  Step s of the
+// reference maps its value X_s = floor(x / 2^(30 s)) to X_{s+1} = floor(X_s / 2^j): with its
+// residues y (as the reference holds them, possibly y + m or y - m) it forms c = y * w (for the
+// rank), X_s mod 2^j, multiple = (y - (X_s mod 2^j) mod m + m) % m and y' = multiple * (2^j)^-1,
+// the products by reciprocal reduction.  Whenever every c is canonical, the rank and the residue
+// X_s mod 2^j of the reference are consistent, so X_s mod 2^j is the word bits [30 s, 30 s + j)
+// of x, and y' follows from y exactly as the reference computes it.  A step whose c is not
+// canonical or whose y' wraps int32 (the reference then continues with another number) sets
+// `replay_ok` false and the caller runs the reference steps instead.
+template <int GD>
+GRNS_D int group_scale2pow_words(const DevCtx *c, const Group &g, int xi, unsigned D, bool &replay_ok) {
+    const bool act = g.gl < g.L;
+    const int m = act ? c->m[g.gl] : 1;
+    const double dinv = act ? c->dinv[g.gl] : 1.0;
+    const int w = act ? c->w[g.gl] : 0;
+    const unsigned X = group_words<GD>(c, g, xi);
+    const int xbits = 32 * wlen<GD>(g, X);
+    const unsigned t = D / THRESH, dr = D % THRESH;
+    const unsigned steps = t + (dr ? 1u : 0u);
+    int y = xi;
+    bool bad = false;
+    for (unsigned st = 0; st < steps; st++) {
+        const long long b = (long long)st * THRESH;
+        if (b >= xbits && group_all(g, !act || y == 0)) break;  // every later step maps 0 to 0
+        const int j = (st < t) ? THRESH : (int)dr;
+        unsigned r = 0;  // bits [b, b + j) of x
+        if (b < xbits) {
+            const int w0 = (int)(b >> 5), sh = (int)(b & 31);
+            const unsigned lo = gword<GD>(g, X, w0);
+            const unsigned hi = (sh && w0 + 1 < GD) ? gword<GD>(g, X, w0 + 1) : 0u;
+            r = (sh ? (lo >> sh) | (hi << (32 - sh)) : lo) & ((1u << j) - 1u);
+        }
+        if (act) {
+            const int cc = mulmod_recip(y, w, m, dinv);
+            bad |= (unsigned)cc >= (unsigned)m;
+            const long long rm = rem_trunc((long long)r, m, dinv);
+            const int mult = (int)rem_trunc((long long)y - rm + m, m, dinv);
+            const int inv = c->pow2_inv[j - 1][g.gl];
+            const int qi = (int)(((double)mult * (double)inv) * dinv);
+            const long long v = (long long)mult * inv - (long long)qi * m;
+            bad |= v < -2147483648ll || v > 2147483647ll;
+            y = (int)v;
+        }
+    }
+    replay_ok = !group_any(g, bad);
+    return y;
+}
+
 GRNS_D er_t bcast_er(er_t v, int lane) {
     er_t r;
     r.frac = bcast_d(v.frac, lane);

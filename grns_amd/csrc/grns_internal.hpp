@@ -37,7 +37,7 @@ struct grns_ctx {
         long long launches = 0;
     };
     bool timing = false;
-    bool reference_loops = false;  // division: the reference loop for every pair (test hook)
+    bool reference_loops = false;  // division and scaling: the reference loops for every number (test hook)
     StageTimer timers[3];
     std::vector<int> pow2;
     std::vector<int> mrc;

@@ -1298,6 +1298,57 @@ GRNS_D int group_scale2pow(const DevCtx *c, const Group &g, int xi, unsigned D) 
     return result;
 }
 
+// Scaling of canonical numbers by the replayed word form; the others (non-canonical residues, a
+// replay that meets an int32 wrap) go to the rare queue for the reference steps.
+template <int GD>
+__global__ __launch_bounds__(BLOCK) void k_scale2pow_words(const DevCtx *__restrict__ c, const int *__restrict__ x,
+                                                           long long smod, long long snum,
+                                                           const unsigned *__restrict__ D, long long count,
+                                                           int *__restrict__ y, RareQ rq) {
+    const Group g = make_group(GD, c->n);
+    constexpr int P = 64 / GD;
+    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
+    for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
+        const long long j = j0 + (threadIdx.x & 63) / GD;
+        const bool valid = j < count;
+        bool ok = false;
+        if (valid) {
+            const bool act = g.gl < g.L;
+            const long long off = j * snum + (long long)g.gl * smod;
+            const int xi = act ? x[off] : 0;
+            if (group_all(g, !act || (unsigned)xi < (unsigned)c->m[g.gl])) {
+                const int v = group_scale2pow_words<GD>(c, g, xi, D[j], ok);
+                if (ok && act) y[off] = v;
+            }
+        }
+        const bool rare = valid && !ok && g.gl == 0;
+        if (!rare_push(rq, rare, j)) y[j * snum] = DIV_MARK;
+    }
+}
+
+// Rare pass of the scaling: the reference steps, one wave group per queued number (and, after a
+// queue overflow, per marked number).
+static __global__ __launch_bounds__(BLOCK) void k_scale2pow_rare(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
+                                                          long long smod, long long snum, const unsigned *__restrict__ D,
+                                                          long long count, int *__restrict__ y, RareQ rq) {
+    Group g = make_group(G, c->n);
+    const int P = 64 / G;
+    const long long gid = (((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6) * P + (threadIdx.x & 63) / G;
+    const long long ngroups = ((long long)gridDim.x * BLOCK >> 6) * P;
+    const unsigned pushed = rq.ctr[0];
+    const long long nq = pushed < rq.cap ? pushed : rq.cap;
+    const long long total = nq + (rq.ctr[2] ? count : 0);
+    for (long long k = gid; k < total; k += ngroups) {
+        const long long j = rare_item(rq, nq, k, total);
+        if (k >= nq && y[j * snum] != DIV_MARK) continue;
+        const long long off = j * snum + (long long)g.gl * smod;
+        const int xi = g.gl < g.L ? x[off] : 0;
+        const int v = group_scale2pow(c, g, xi, D[j]);
+        if (g.gl < g.L) y[off] = v;
+    }
+}
+
 static __global__ __launch_bounds__(BLOCK) void k_scale2pow(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
                                                      long long smod, long long snum, const unsigned *__restrict__ D,
                                                      long long count, int *__restrict__ y,

@@ -562,7 +562,29 @@ grns_status grns_scale2pow(grns_ctx *c, const int *x, long long smod, long long 
     DeviceGuard guard(c->device);
     int G = groups_for(c->n);
     dim3 grid((unsigned)grid_groups(count, G)), blk(BLOCK);
-    hipLaunchKernelGGL(k_scale2pow, grid, blk, 0, S(stream), c->d, G, x, smod, snum, D, (long long)count, y,
+    const long long cnt = (long long)count;
+    // canonical numbers: the reference's steps replayed on words; the reference steps for the others
+    const int GW = c->reference_loops ? 0 : words_group(c);
+    if (GW) {
+        RareQ rq;
+        grns_status qs = rare_queue(c, stream, rq);
+        if (qs != GRNS_OK) return qs;
+        dim3 gw((unsigned)grid_groups(count, GW)), rg((unsigned)rare_grid(count));
+        switch (GW) {
+            case 2: hipLaunchKernelGGL((k_scale2pow_words<2>), gw, blk, 0, S(stream), c->d, x, smod, snum, D, cnt, y, rq); break;
+            case 4: hipLaunchKernelGGL((k_scale2pow_words<4>), gw, blk, 0, S(stream), c->d, x, smod, snum, D, cnt, y, rq); break;
+            case 8: hipLaunchKernelGGL((k_scale2pow_words<8>), gw, blk, 0, S(stream), c->d, x, smod, snum, D, cnt, y, rq); break;
+            case 16: hipLaunchKernelGGL((k_scale2pow_words<16>), gw, blk, 0, S(stream), c->d, x, smod, snum, D, cnt, y, rq); break;
+            case 32: hipLaunchKernelGGL((k_scale2pow_words<32>), gw, blk, 0, S(stream), c->d, x, smod, snum, D, cnt, y, rq); break;
+            default: hipLaunchKernelGGL((k_scale2pow_words<64>), gw, blk, 0, S(stream), c->d, x, smod, snum, D, cnt, y, rq); break;
+        }
+        if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;
+        hipLaunchKernelGGL(k_scale2pow_rare, rg, blk, 0, S(stream), c->d, G, x, smod, snum, D, cnt, y, rq);
+        if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;
+        hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), rq.ctr);
+        return launch_check();
+    }
+    hipLaunchKernelGGL(k_scale2pow, grid, blk, 0, S(stream), c->d, G, x, smod, snum, D, cnt, y,
                        (const long long *)nullptr);
     return launch_check();
 }

@@ -226,6 +226,50 @@ def test_div_noncanonical_residues(ctxs, dev, n):
     assert c.status() == 0
 
 
+@pytest.mark.parametrize("n,count", [(8, 1 << 18), (32, 1 << 16), (64, 1 << 14)])
+def test_scale2pow_words_against_reference_steps(ctxs, dev, n, count):
+    """The replayed word scaling equals the reference steps run on the device, for exponents from
+    0 to past the range (small results make the reference's products wrap: those numbers take the
+    reference steps)."""
+    import grns_amd as G
+    from grns_amd.workload import Synth
+    c, o, mod = ctxs[n]
+    L = G.product(mod).bit_length()
+    syn = Synth(mod)
+    x = syn.uniform(count, 900 + n, 1)
+    D = syn.uniform_u32(count, 0, L + 70, 900 + n, 2)
+    y1, y2 = torch.empty_like(x), torch.empty_like(x)
+    c.scale2pow(x, D, y=y1)
+    c.reference_loops(True)
+    try:
+        c.scale2pow(x, D, y=y2)
+    finally:
+        c.reference_loops(False)
+    assert torch.equal(y1, y2)
+
+
+@pytest.mark.parametrize("n", [4, 8, 32, 64])
+def test_scale2pow_structured(ctxs, dev, n):
+    """Scaling of values at fixed fractions of M, powers of two and word boundaries by exponents
+    around the range, against the CPU path."""
+    from grns_amd import product
+    c, o, mod = ctxs[n]
+    M = product(mod)
+    L = M.bit_length()
+    rnd = random.Random(95 + n)
+    xs = [M * k // 100000 for k in (9104, 8450, 23150, 50000, 99900, 1, 77777)]
+    xs += [(1 << k) - 1 for k in range(1, L - 1, 7)] + [1 << k for k in range(0, L - 1, 11)] + [M - 1, 0, 1]
+    xs = [v for v in xs for _ in range(3)]
+    Ds = [max(0, rnd.choice([0, 1, 29, 30, 31, 60, L - 30, L - 9, L - 7, L - 5, L - 2, L - 1, L, L + 3, 5000]))
+          for _ in xs]
+    x = to_res(xs, mod)
+    D = np.array(Ds, np.uint32)
+    y = c.scale2pow(_dev(x, dev), _dev(D.view(np.int32), dev)).cpu().numpy()
+    ref = o.scale2pow_array(x, D)
+    bad = np.nonzero((y != ref).any(axis=0))[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first x={xs[bad[0]]} D={Ds[bad[0]]}"
+
+
 @pytest.mark.parametrize("n", [8, 32, 64])
 def test_scale2pow_parity(ctxs, dev, n):
     c, o, mod = ctxs[n]
