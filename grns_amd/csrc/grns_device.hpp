@@ -170,6 +170,13 @@ GRNS_HD double bitsd(uint64_t u) {
     union { double d; uint64_t u; } v; v.u = u; return v.d;
 }
 
+// For a positive c >= 2^-960, c - bn_err(c) is the next double below c and c + bn_err(c) the next
+// above: bn_err(c) = RN(PHI1 c + ETA) lies in (ulp(c)/2, ulp(c)] (PHI1 = 2^-53 (1 + 2^-52), ETA far
+// below ulp(PHI1 c)), so the rounded difference and sum are c's neighbours.  One 64-bit integer
+// step on the bit pattern gives the same doubles as dsub_rd / dadd_ru of the bound.
+GRNS_HD double below_pos(double c) { return bitsd(dbits(c) - 1); }
+GRNS_HD double above_pos(double c) { return bitsd(dbits(c) + 1); }
+
 // fast_scalbn, src/bitwise.cuh:75-80: add n to the exponent field, then multiply by 0 or 1.
 // The multiply by zero of an infinity or NaN produced by the exponent wrap is spelled out with
 // the x86 SSE results (default NaN 0xFFF8..., input NaN quieted), so that device and CPU agree.
@@ -369,8 +376,12 @@ struct TreeWidth {
     static constexpr int v = calc();
 };
 
-template <int K, int L, int OFF>
+// POS: every partial sum is known to be a positive normal double (the leaves are), so the bounded
+// additions are neighbour steps (below_pos / above_pos).
+template <int K, int L, int OFF, bool POS = false>
 struct PTree {
+    GRNS_HD static double add_rd(double a, double b) { return POS ? below_pos(a + b) : dadd_rd(a, b); }
+    GRNS_HD static double add_ru(double a, double b) { return POS ? above_pos(a + b) : dadd_ru(a, b); }
     template <class Leaf>
     GRNS_HD static void run(Leaf &leaf, double &lo, double &up) {
         if constexpr (K == 2) {
@@ -379,27 +390,27 @@ struct PTree {
             if constexpr (L > 1) {
                 double l1, u1;
                 leaf.template get<OFF + 1>(l1, u1);
-                lo = dadd_rd(l0, l1);
-                up = dadd_ru(u0, u1);
+                lo = add_rd(l0, l1);
+                up = add_ru(u0, u1);
             } else {
-                lo = dadd_rd(l0, 0.0);
-                up = dadd_ru(u0, 0.0);
+                lo = add_rd(l0, 0.0);
+                up = add_ru(u0, 0.0);
             }
         } else {
             double l1, u1, l2 = 0.0, u2 = 0.0;
-            PTree<K / 2, L, OFF>::run(leaf, l1, u1);
-            if constexpr (L > K / 2) PTree<K / 2, L - K / 2, OFF + K / 2>::run(leaf, l2, u2);
-            lo = dadd_rd(l1, l2);
-            up = dadd_ru(u1, u2);
+            PTree<K / 2, L, OFF, POS>::run(leaf, l1, u1);
+            if constexpr (L > K / 2) PTree<K / 2, L - K / 2, OFF + K / 2, POS>::run(leaf, l2, u2);
+            lo = add_rd(l1, l2);
+            up = add_ru(u1, u2);
         }
     }
 };
 
-// psum_rd<N> and psum_ru<N> together (src/pairwise.cuh:384-458 dispatch).
-template <int N, class Leaf>
+// psum_rd<N> and psum_ru<N> together (src/pairwise.cuh:384-458 dispatch).  POS as for PTree.
+template <int N, class Leaf, bool POS = false>
 GRNS_HD void psum_pair(Leaf &leaf, double &lo, double &up) {
     if constexpr (N >= 4) {
-        PTree<TreeWidth<N>::v, N, 0>::run(leaf, lo, up);
+        PTree<TreeWidth<N>::v, N, 0, POS>::run(leaf, lo, up);
     } else if constexpr (N == 3) {
         double l0, u0, l1, u1, l2, u2;
         leaf.template get<0>(l0, u0);

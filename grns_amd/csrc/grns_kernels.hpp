@@ -48,7 +48,7 @@ struct EvalLeaf {
     const int *xr;
     // the trees ask for leaves in ascending order: the constants come in views of 4 moduli
     cdouble *dw = nullptr, *dm = nullptr, *dinv = nullptr, *rd = nullptr, *ru = nullptr;
-    double smax = 0.0;  // max |s_i| before the int32 wrap: >= 2^31 leaves the number to the rare pass
+    double smax = 0.0;  // max |s_i - 2^30|: >= 2^30 (s_i = 0, negative or wrapping) leaves the number to the rare pass
     template <int I>
     GRNS_HD void get(double &lo, double &up) {
         if constexpr (I % 4 == 0) {
@@ -61,9 +61,11 @@ struct EvalLeaf {
         }
         constexpr int k = I % 4;
         double s = mulmod_recip_d_nowrap((double)xr[I], dw[k], dm[k], dinv[k]);
-        smax = fmax(smax, fabs(s));
-        lo = dmul_rd(s, rd[k]);
-        up = dmul_ru(s, ru[k]);
+        // s in [1, 2^31): the reference's int result is s itself and both products are at least
+        // 2^-32, so their bounds (and those of every partial sum) are the neighbouring doubles
+        smax = fmax(smax, fabs(s - 0x1p30));
+        lo = below_pos(s * rd[k]);
+        up = above_pos(s * ru[k]);
     }
 };
 
@@ -76,7 +78,7 @@ GRNS_HD bool eval_hot(const DevCtx *c, const int (&xr)[N], interval_t &out) {
     for (int i = 0; i < N; i++) nz |= xr[i];
     EvalLeaf<N> leaf{c, xr};
     double suml, sumu;
-    psum_pair<N>(leaf, suml, sumu);
+    psum_pair<N, EvalLeaf<N>, true>(leaf, suml, sumu);
     unsigned whl = to_uint(suml), whu = to_uint(sumu);
     suml -= whl;
     sumu -= whu;
@@ -86,7 +88,7 @@ GRNS_HD bool eval_hot(const DevCtx *c, const int (&xr)[N], interval_t &out) {
         out.low.frac = 0; out.low.exp = 0; out.upp.frac = 0; out.upp.exp = 0;
         return true;
     }
-    return leaf.smax < 2147483648.0 && (FAST || whl == whu) && (sumu >= c->accuracy);
+    return leaf.smax < 0x1p30 && (FAST || whl == whu) && (sumu >= c->accuracy);
 }
 
 GRNS_D void store_interval(interval_t *p, const interval_t &v) {
