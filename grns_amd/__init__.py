@@ -70,7 +70,8 @@ EXPORTS = [
     "grns_max_workspace_size", "grns_max", "grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul",
     "grns_batch_create", "grns_batch_set_config", "grns_batch_get_config", "grns_batch_tune", "grns_batch_launch",
     "grns_batch_destroy", "grns_serve_create", "grns_serve_submit", "grns_serve_poll", "grns_serve_wait",
-    "grns_serve_drain", "grns_serve_stats_get", "grns_serve_destroy",
+    "grns_serve_drain", "grns_serve_stats_get", "grns_serve_destroy", "grns_ctx_words", "grns_from_binary",
+    "grns_to_binary",
 ]
 DEBUG_EXPORTS = [
     "grns_debug_eval_hot", "grns_debug_filter", "grns_debug_cmp_lane", "grns_debug_max_key", "grns_debug_ceil_log2",
@@ -105,6 +106,9 @@ def lib():
     L.grns_div.argtypes = [P, P, P, LL, LL, SZ, P, P, I, P]
     L.grns_scale2pow.argtypes = [P, P, LL, LL, P, SZ, P, P]
     L.grns_max_workspace_size.restype = SZ
+    L.grns_ctx_words.argtypes = [P]
+    L.grns_from_binary.argtypes = [P, P, LL, LL, I, P, SZ, P, LL, LL, P]
+    L.grns_to_binary.argtypes = [P, P, LL, LL, SZ, P, LL, LL, P]
     L.grns_max_workspace_size.argtypes = [P, SZ]
     L.grns_max.argtypes = [P, P, LL, LL, SZ, P, P, P, SZ, P]
     for nm in ("grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul"):
@@ -282,6 +286,33 @@ class Context:
         _check(lib().grns_scale2pow(self._h, _tptr(x), sm, sn, _tptr(D), x.shape[1], _tptr(y),
                                     _stream_ptr(stream)), "grns_scale2pow")
         return y
+
+    @property
+    def words(self) -> int:
+        """32-bit words of M (the length grns_to_binary writes)."""
+        return lib().grns_ctx_words(self._h)
+
+    def from_binary(self, w, sign=None, out=None, stream=None):
+        """Residues (n, count) of the integers whose little-endian 32-bit words are w[j, k]
+        (an int32 (count, nwords) tensor holding the bit patterns); sign[j] != 0 marks negatives."""
+        import torch
+        count, nw = w.shape
+        out = torch.empty((self.n, count), dtype=torch.int32, device=w.device) if out is None else out
+        sm, sn = self._strides(out)
+        _check(lib().grns_from_binary(self._h, _tptr(w), w.stride(1), w.stride(0), nw,
+                                      _tptr(sign) if sign is not None else None, count, _tptr(out), sm, sn,
+                                      _stream_ptr(stream)), "grns_from_binary")
+        return out
+
+    def to_binary(self, x, out=None, stream=None):
+        """Words (count, words) of the CRT values of x (int32 bit patterns of the uint32 words)."""
+        import torch
+        count = x.shape[1]
+        out = torch.empty((count, self.words), dtype=torch.int32, device=x.device) if out is None else out
+        sm, sn = self._strides(x)
+        _check(lib().grns_to_binary(self._h, _tptr(x), sm, sn, count, _tptr(out), out.stride(1), out.stride(0),
+                                    _stream_ptr(stream)), "grns_to_binary")
+        return out
 
     def max_workspace_size(self, count) -> int:
         return lib().grns_max_workspace_size(self._h, count)

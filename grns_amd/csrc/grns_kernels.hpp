@@ -1228,6 +1228,74 @@ __global__ __launch_bounds__(BLOCK) void k_div(const DevCtx *__restrict__ c, int
     }
 }
 
+// ---------------------------------------------------------------- binary <-> RNS conversion
+// rns_from_binary (src/rnsbase.cuh:253-261): lane i of a group forms |x| mod m_i by Horner's rule
+// over the words, from the most significant (word k read once per group, GD words per load).
+template <int GD>
+__global__ __launch_bounds__(BLOCK) void k_from_binary(const DevCtx *__restrict__ c, const unsigned *__restrict__ w,
+                                                       long long wst, long long wsn, int nwords,
+                                                       const int *__restrict__ sign, long long count,
+                                                       int *__restrict__ x, long long smod, long long snum) {
+    const Group g = make_group(GD, c->n);
+    constexpr int P = 64 / GD;
+    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
+    const bool act = g.gl < g.L;
+    const int m = act ? c->m[g.gl] : 1;
+    const double dinv = act ? c->dinv[g.gl] : 1.0;
+    const long long c32 = act ? c->c32[g.gl] : 0;
+    for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
+        const long long j = j0 + (threadIdx.x & 63) / GD;
+        if (j >= count) continue;
+        long long acc = 0;
+        const int top = (nwords - 1) / GD;
+        for (int ch = top; ch >= 0; ch--) {  // chunks of GD words, most significant first
+            const int k0 = ch * GD;
+            const int kn = nwords - k0 < GD ? nwords - k0 : GD;
+            const unsigned wk = g.gl < kn ? w[j * wsn + (long long)(k0 + g.gl) * wst] : 0u;
+            for (int k = kn - 1; k >= 0; k--) {
+                const unsigned v = gword<GD>(g, wk, k);
+                acc = rem_trunc(acc * c32 + (long long)v, m, dinv);
+            }
+        }
+        if (act) {
+            if (sign && sign[j] && acc != 0) acc = m - acc;
+            x[j * snum + (long long)g.gl * smod] = (int)acc;
+        }
+    }
+}
+
+// rns_to_binary (src/rnsbase.cuh:277-287): each residue reduced as the reference's mpz_mul_ui
+// sees it (an int cast to unsigned long: x_i + 2^64 when negative), then the words of the CRT value.
+template <int GD>
+__global__ __launch_bounds__(BLOCK) void k_to_binary(const DevCtx *__restrict__ c, const int *__restrict__ x,
+                                                     long long smod, long long snum, long long count,
+                                                     unsigned *__restrict__ w, long long wst, long long wsn) {
+    const Group g = make_group(GD, c->n);
+    constexpr int P = 64 / GD;
+    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
+    const bool act = g.gl < g.L;
+    const int m = act ? c->m[g.gl] : 1;
+    const double dinv = act ? c->dinv[g.gl] : 1.0;
+    const long long c32 = act ? c->c32[g.gl] : 0;
+    const long long p64 = act ? rem_trunc(c32 * c32, m, dinv) : 0;  // 2^64 mod m
+    const int W = c->words - 1;
+    for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
+        const long long j = j0 + (threadIdx.x & 63) / GD;
+        if (j >= count) continue;
+        int xi = 0;
+        if (act) {
+            const long long v = x[j * snum + (long long)g.gl * smod];
+            long long r = rem_trunc(v, m, dinv);
+            if (v < 0) r = rem_trunc(r + p64 + m, m, dinv);
+            xi = (int)r;
+        }
+        const unsigned X = group_words<GD>(c, g, xi);
+        if (g.gl < W) w[j * wsn + (long long)g.gl * wst] = X;
+    }
+}
+
 // ---------------------------------------------------------------- scaling by 2^D (group per number)
 // rns_rank_compute / _fast, src/rnsscal.cuh:17-62
 GRNS_D int group_rank(const DevCtx *c, const Group &g, int xi, int s) {

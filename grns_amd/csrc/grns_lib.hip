@@ -377,6 +377,7 @@ grns_status grns_ctx_destroy(grns_ctx *c) {
 
 int grns_ctx_n(const grns_ctx *c) { return c ? c->n : 0; }
 int grns_ctx_log2m(const grns_ctx *c) { return c ? c->log2M : 0; }
+int grns_ctx_words(const grns_ctx *c) { return c ? c->h.words - 1 : 0; }
 
 grns_status grns_ctx_constants(const grns_ctx *c, int *part_inv, double *recip_rd, double *recip_ru,
                                double *accuracy, int *ref_factor, grns_interval *unit, grns_interval *inv_unit) {
@@ -550,6 +551,49 @@ grns_status grns_div(grns_ctx *c, const int *x, const int *d, long long smod, lo
     }
     if (fast) hipLaunchKernelGGL((k_div<true>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, (const long long *)nullptr, c->d_status);
     else hipLaunchKernelGGL((k_div<false>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, (const long long *)nullptr, c->d_status);
+    return launch_check();
+}
+
+grns_status grns_from_binary(grns_ctx *c, const unsigned *w, long long wst, long long wsn, int nwords, const int *sign,
+                             size_t count, int *x, long long smod, long long snum, void *stream) {
+    if (c && c->device < 0) return GRNS_ERR_ARG;
+    if (!c || nwords < 1 || (count && (!w || !x))) return GRNS_ERR_ARG;
+    if (count == 0) return GRNS_OK;
+    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
+    DeviceGuard guard(c->device);
+    const int G = groups_for(c->n);
+    dim3 grid((unsigned)grid_groups(count, G)), blk(BLOCK);
+    const long long cnt = (long long)count;
+    switch (G) {
+        case 1: hipLaunchKernelGGL((k_from_binary<1>), grid, blk, 0, S(stream), c->d, w, wst, wsn, nwords, sign, cnt, x, smod, snum); break;
+        case 2: hipLaunchKernelGGL((k_from_binary<2>), grid, blk, 0, S(stream), c->d, w, wst, wsn, nwords, sign, cnt, x, smod, snum); break;
+        case 4: hipLaunchKernelGGL((k_from_binary<4>), grid, blk, 0, S(stream), c->d, w, wst, wsn, nwords, sign, cnt, x, smod, snum); break;
+        case 8: hipLaunchKernelGGL((k_from_binary<8>), grid, blk, 0, S(stream), c->d, w, wst, wsn, nwords, sign, cnt, x, smod, snum); break;
+        case 16: hipLaunchKernelGGL((k_from_binary<16>), grid, blk, 0, S(stream), c->d, w, wst, wsn, nwords, sign, cnt, x, smod, snum); break;
+        case 32: hipLaunchKernelGGL((k_from_binary<32>), grid, blk, 0, S(stream), c->d, w, wst, wsn, nwords, sign, cnt, x, smod, snum); break;
+        default: hipLaunchKernelGGL((k_from_binary<64>), grid, blk, 0, S(stream), c->d, w, wst, wsn, nwords, sign, cnt, x, smod, snum); break;
+    }
+    return launch_check();
+}
+
+grns_status grns_to_binary(grns_ctx *c, const int *x, long long smod, long long snum, size_t count, unsigned *w,
+                           long long wst, long long wsn, void *stream) {
+    if (c && c->device < 0) return GRNS_ERR_ARG;
+    if (!c || (count && (!w || !x))) return GRNS_ERR_ARG;
+    if (count == 0) return GRNS_OK;
+    const int GW = words_group(c);
+    if (c->n > 64 || !GW) return GRNS_ERR_UNSUPPORTED;
+    DeviceGuard guard(c->device);
+    dim3 grid((unsigned)grid_groups(count, GW)), blk(BLOCK);
+    const long long cnt = (long long)count;
+    switch (GW) {
+        case 2: hipLaunchKernelGGL((k_to_binary<2>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, w, wst, wsn); break;
+        case 4: hipLaunchKernelGGL((k_to_binary<4>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, w, wst, wsn); break;
+        case 8: hipLaunchKernelGGL((k_to_binary<8>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, w, wst, wsn); break;
+        case 16: hipLaunchKernelGGL((k_to_binary<16>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, w, wst, wsn); break;
+        case 32: hipLaunchKernelGGL((k_to_binary<32>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, w, wst, wsn); break;
+        default: hipLaunchKernelGGL((k_to_binary<64>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, w, wst, wsn); break;
+    }
     return launch_check();
 }
 

@@ -84,6 +84,21 @@ grns_status grns_div(grns_ctx *ctx, const int *x, const int *d, long long mod_st
 grns_status grns_scale2pow(grns_ctx *ctx, const int *x, long long mod_stride, long long num_stride,
                            const unsigned *D, size_t count, int *y, void *stream);
 
+/* Binary <-> RNS conversion.  Integers are little-endian 32-bit words: word k of number j at
+ * w[j * num_stride + k * word_stride].
+ * grns_from_binary: rns_from_binary (src/rnsbase.cuh:253-261, mpz_mod_ui per modulus): residue i
+ * of number j is |x_j| mod m_i, and m_i minus that (when nonzero) if sign[j] != 0; sign may be
+ * NULL (all nonnegative); nwords >= 1 words per number, any length.
+ * grns_to_binary: rns_to_binary (src/rnsbase.cuh:277-287): the CRT value 0 <= X < M of each number
+ * as grns_ctx_words(ctx) words; a negative residue contributes as the reference's unsigned long
+ * conversion of it does (x_i + 2^64). */
+int grns_ctx_words(const grns_ctx *ctx);
+grns_status grns_from_binary(grns_ctx *ctx, const unsigned *w, long long word_stride, long long num_stride,
+                             int nwords, const int *sign, size_t count, int *x, long long mod_stride,
+                             long long num_stride_x, void *stream);
+grns_status grns_to_binary(grns_ctx *ctx, const int *x, long long mod_stride, long long num_stride_x, size_t count,
+                           unsigned *w, long long word_stride, long long num_stride, void *stream);
+
 /* Arg-max of `count` numbers: cuda::rns_max (src/dpp/rnsmax.cuh:129-138).  out_residues[n]
  * receives the residues of the maximum, out_idx its index (the lowest index among equal
  * maxima), both in device memory.  workspace: device memory of grns_max_workspace_size bytes. */

@@ -414,3 +414,60 @@ def test_scale2pow_large_exponents(ctxs, dev, n):
     D_huge = np.array([rnd.choice([1 << 31, 0xFFFFFFFF, rnd.randrange(1 << 20, 1 << 32)]) for _ in vals], np.uint32)
     y = c.scale2pow(_dev(x, dev), _dev(D_huge.view(np.int32), dev))
     assert (y.cpu().numpy() == 0).all()
+
+
+def _words(v, nw):
+    return [(v >> (32 * k)) & 0xFFFFFFFF for k in range(nw)]
+
+
+@pytest.mark.parametrize("n", [4, 8, 32, 64])
+def test_from_binary(ctxs, dev, n):
+    """rns_from_binary: residues of integers of any length (beyond M and beyond a group's width),
+    with signs (mpz_mod_ui's nonnegative remainder), in number-major and word-major layouts."""
+    c, o, mod = ctxs[n]
+    from grns_amd import product
+    M = product(mod)
+    rnd = random.Random(200 + n)
+    nw = max(3, (M.bit_length() + 31) // 32 + 70)  # longer than M and than 64 words
+    vals = [rnd.getrandbits(rnd.randrange(1, 32 * nw)) for _ in range(700)]
+    vals += [0, 1, M - 1, M, M + 1, 2 ** (32 * nw) - 1, 2 ** 32, 2 ** 32 - 1] + [m * 12345 for m in mod[:3]]
+    signs = [rnd.randrange(2) for _ in vals]
+    W = np.array([_words(v, nw) for v in vals], dtype=np.uint64).astype(np.uint32).view(np.int32)
+    S = np.array(signs, np.int32)
+    got = c.from_binary(_dev(W, dev), sign=_dev(S, dev)).cpu().numpy()
+    exp = to_res([-v if s else v for v, s in zip(vals, signs)], mod)
+    assert (got == exp).all()
+    got2 = c.from_binary(_dev(W.T.copy(), dev).t()).cpu().numpy()  # word-major storage, no signs
+    assert (got2 == to_res(vals, mod)).all()
+
+
+@pytest.mark.parametrize("n", [4, 8, 32, 64])
+def test_to_binary(ctxs, dev, n):
+    """rns_to_binary: CRT values as words, for canonical, non-canonical and negative residues
+    (a negative residue counts as x_i + 2^64, the reference's conversion to unsigned long)."""
+    c, o, mod = ctxs[n]
+    from grns_amd import product
+    M = product(mod)
+    rnd = random.Random(300 + n)
+    vals = [rnd.randrange(M) for _ in range(600)] + [0, 1, M - 1, M // 2]
+    x = to_res(vals, mod)
+    got = c.to_binary(_dev(x, dev)).cpu().numpy().view(np.uint32)
+    W = c.words
+    assert got.shape == (len(vals), W)
+    assert [sum(int(w) << (32 * k) for k, w in enumerate(row)) for row in got] == vals
+    xr = x.astype(np.int64)
+    for j in range(0, x.shape[1], 3):  # raise some residues by m (same value) or make them negative
+        i = rnd.randrange(len(mod))
+        if xr[i, j] + mod[i] < 2 ** 31:
+            xr[i, j] += mod[i]
+        else:
+            xr[i, j] -= mod[i]
+    got = c.to_binary(_dev(xr.astype(np.int32), dev)).cpu().numpy().view(np.uint32)
+    Mi = [M // m for m in mod]
+    B = [Mi[i] * pow(Mi[i] % mod[i], -1, mod[i]) for i in range(len(mod))]
+    exp = [sum(((int(v) + 2 ** 64) if v < 0 else int(v)) * B[i] for i, v in enumerate(xr[:, j])) % M
+           for j in range(x.shape[1])]
+    assert [sum(int(w) << (32 * k) for k, w in enumerate(row)) for row in got] == exp
+    # round trip through the words
+    back = c.from_binary(c.to_binary(_dev(x, dev))).cpu().numpy()
+    assert (back == x).all()
