@@ -211,11 +211,17 @@ def div_scale_leg(steps, warmup, n_mod=64, count=1 << 20, seed=3):
     ms_df = timed(lambda: ctx.div(x, d, fast=True, q=q, r=r), steps, warmup)
     ms_d = timed(lambda: ctx.div(x, d, fast=False, q=q, r=r), max(1, steps // 4), 1)
     ms_s = timed(lambda: ctx.scale2pow(x, D, y=y), steps, warmup)
+    wb = ctx.to_binary(x)
+    if not torch.equal(ctx.from_binary(wb), x):
+        raise SystemExit("binary round trip differs")
+    ms_tb = timed(lambda: ctx.to_binary(x, out=wb), steps, warmup)
+    ms_fb = timed(lambda: ctx.from_binary(wb, out=y), steps, warmup)
     return {"workload": f"n={n_mod} (largest primes below 2^31) N=2^{count.bit_length() - 1} pairs, divisor bits "
                         "uniform in [1, 1980], exponents uniform in [1, 1900]",
             "div_fast_ops_per_s": round(count / (ms_df * 1e-3)), "div_fast_ms": round(ms_df, 4),
             "div_ops_per_s": round(count / (ms_d * 1e-3)), "div_ms": round(ms_d, 4),
             "scale2pow_ops_per_s": round(count / (ms_s * 1e-3)), "scale2pow_ms": round(ms_s, 4),
+            "to_binary_ms": round(ms_tb, 4), "from_binary_ms": round(ms_fb, 4),
             "sample_checked_exact": 256}
 
 

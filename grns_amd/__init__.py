@@ -67,7 +67,7 @@ _lib = None
 EXPORTS = [
     "grns_version", "grns_ctx_create", "grns_ctx_destroy", "grns_ctx_n", "grns_ctx_log2m", "grns_ctx_constants",
     "grns_ctx_tables", "grns_ctx_status", "grns_eval", "grns_cmp", "grns_div", "grns_scale2pow",
-    "grns_max_workspace_size", "grns_max", "grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul",
+    "grns_max_workspace_size", "grns_max", "grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul", "grns_mpint_div",
     "grns_batch_create", "grns_batch_set_config", "grns_batch_get_config", "grns_batch_tune", "grns_batch_launch",
     "grns_batch_destroy", "grns_serve_create", "grns_serve_submit", "grns_serve_poll", "grns_serve_wait",
     "grns_serve_drain", "grns_serve_stats_get", "grns_serve_destroy", "grns_ctx_words", "grns_from_binary",
@@ -111,7 +111,7 @@ def lib():
     L.grns_to_binary.argtypes = [P, P, LL, LL, SZ, P, LL, LL, P]
     L.grns_max_workspace_size.argtypes = [P, SZ]
     L.grns_max.argtypes = [P, P, LL, LL, SZ, P, P, P, SZ, P]
-    for nm in ("grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul"):
+    for nm in ("grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul", "grns_mpint_div"):
         getattr(L, nm).argtypes = [P, C.POINTER(MpintArray), C.POINTER(MpintArray), C.POINTER(MpintArray), SZ, P, P]
     L.grns_batch_create.argtypes = [P, C.POINTER(BatchTask), I, C.POINTER(P)]
     L.grns_batch_set_config.argtypes = [P, C.POINTER(BatchConfig)]
@@ -332,8 +332,9 @@ class Context:
         return out, out_idx
 
     def mpint(self, op, x, y, z, count, overflow=None, stream=None):
-        """op in {'add','sub','mul'}; x, y, z: (digits, sign, eval) device tensor triples."""
-        fn = {"add": lib().grns_mpint_add, "sub": lib().grns_mpint_sub, "mul": lib().grns_mpint_mul}[op]
+        """op in {'add','sub','mul','div'}; x, y, z: (digits, sign, eval) device tensor triples."""
+        fn = {"add": lib().grns_mpint_add, "sub": lib().grns_mpint_sub, "mul": lib().grns_mpint_mul,
+              "div": lib().grns_mpint_div}[op]
         xa, ya, za = mpint_array(*x), mpint_array(*y), mpint_array(*z)
         _check(fn(self._h, C.byref(xa), C.byref(ya), C.byref(za), count,
                   _tptr(overflow) if overflow is not None else None, _stream_ptr(stream)), f"grns_mpint_{op}")

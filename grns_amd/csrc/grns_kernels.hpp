@@ -1576,4 +1576,44 @@ __global__ __launch_bounds__(BLOCK) void k_mpint(const DevCtx *__restrict__ c, M
         mpint_step<N, OP>(c, A, j);
 }
 
+// mpint_div (src/mpint.cuh:286-296, CPU path) of element j by a group: rns_div_fast of the
+// magnitudes (the word division, or the reference loop for non-canonical digits and a zero
+// divisor), +1 by C's int % for a negative dividend with a nonzero remainder, rns_eval_compute of
+// the digits, sign 0 for a zero quotient and x.sign ^ y.sign otherwise.
+template <int GD>
+GRNS_D void group_mpint_div(const DevCtx *c, const Group &g, const MpintArgs &A, long long j, int &status) {
+    const bool act = g.gl < g.L;
+    const long long off = (long long)g.gl * A.ld + j;
+    const int xi = act ? A.xd[off] : 0, yi = act ? A.yd[off] : 0;
+    const int xs = A.xs[j], ys = A.ys[j];
+    int q, r;
+    if (GD >= c->words && div_words_ok(c, g, xi, yi)) group_div_words<GD>(c, g, xi, yi, q, r);
+    else group_div<true>(c, g, xi, yi, q, r, status);
+    const bool rz = group_all(g, !act || r == 0);
+    if (xs == 1 && !rz && act) q = (int)rem_trunc((long long)q + 1, c->m[g.gl], c->dinv[g.gl]);
+    const interval_t e = group_eval<false>(c, g, q, status);
+    const bool qz = group_all(g, !act || q == 0);
+    if (act) A.rd[off] = q;
+    if (g.gl == 0) {
+        A.rs[j] = qz ? 0 : (xs ^ ys);
+        A.re[2 * j] = e.low;
+        A.re[2 * j + 1] = e.upp;
+    }
+}
+
+template <int GD>
+__global__ __launch_bounds__(BLOCK) void k_mpint_div(const DevCtx *__restrict__ c, MpintArgs A, long long count,
+                                                     int *__restrict__ status) {
+    const Group g = make_group(GD, c->n);
+    constexpr int P = 64 / GD;
+    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
+    int st = 0;
+    for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
+        const long long j = j0 + (threadIdx.x & 63) / GD;
+        if (j < count) group_mpint_div<GD>(c, g, A, j, st);
+    }
+    if (g.gl == 0 && st) atomicOr(status, st);
+}
+
 }  // namespace grns

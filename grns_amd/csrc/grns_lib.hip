@@ -741,6 +741,33 @@ grns_status grns_mpint_mul(grns_ctx *c, const grns_mpint_array *x, const grns_mp
                            const grns_mpint_array *z, size_t count, unsigned char *ovf, void *stream) {
     return mpint_launch(c, 2, x, y, z, count, ovf, stream);
 }
+grns_status grns_mpint_div(grns_ctx *c, const grns_mpint_array *x, const grns_mpint_array *y,
+                           const grns_mpint_array *z, size_t count, unsigned char *ovf, void *stream) {
+    (void)ovf;  // the reference checks no overflow for a division
+    if (!c || !x || !y || !z || c->device < 0) return GRNS_ERR_ARG;
+    if (count == 0) return GRNS_OK;
+    if (x->ld != y->ld || x->ld != z->ld || x->ld < (long long)count) return GRNS_ERR_ARG;
+    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
+    DeviceGuard guard(c->device);
+    MpintArgs A;
+    A.xd = x->digits; A.xs = x->sign; A.xe = reinterpret_cast<const er_t *>(x->eval);
+    A.yd = y->digits; A.ys = y->sign; A.ye = reinterpret_cast<const er_t *>(y->eval);
+    A.rd = z->digits; A.rs = z->sign; A.re = reinterpret_cast<er_t *>(z->eval);
+    A.ovf = nullptr; A.ld = x->ld;
+    const int GW = words_group(c) ? words_group(c) : groups_for(c->n);
+    dim3 grid((unsigned)grid_groups(count, GW)), blk(BLOCK);
+    const long long cnt = (long long)count;
+    switch (GW) {
+        case 1: hipLaunchKernelGGL((k_mpint_div<1>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+        case 2: hipLaunchKernelGGL((k_mpint_div<2>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+        case 4: hipLaunchKernelGGL((k_mpint_div<4>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+        case 8: hipLaunchKernelGGL((k_mpint_div<8>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+        case 16: hipLaunchKernelGGL((k_mpint_div<16>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+        case 32: hipLaunchKernelGGL((k_mpint_div<32>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+        default: hipLaunchKernelGGL((k_mpint_div<64>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+    }
+    return launch_check();
+}
 
 }  // extern "C"
 

@@ -124,6 +124,10 @@ grns_status grns_mpint_sub(grns_ctx *ctx, const grns_mpint_array *x, const grns_
                            const grns_mpint_array *z, size_t count, unsigned char *overflow, void *stream);
 grns_status grns_mpint_mul(grns_ctx *ctx, const grns_mpint_array *x, const grns_mpint_array *y,
                            const grns_mpint_array *z, size_t count, unsigned char *overflow, void *stream);
+/* mpint_div (src/mpint.cuh:286-296): Euclidean quotient z = x / y (digits, sign, eval); the eval
+ * fields of x and y are not read, and overflow (may be NULL) is not written. */
+grns_status grns_mpint_div(grns_ctx *ctx, const grns_mpint_array *x, const grns_mpint_array *y,
+                           const grns_mpint_array *z, size_t count, unsigned char *overflow, void *stream);
 
 /* ---------------------------------------------------------------- batched form (no reference
  * counterpart): many arrays of uneven length and several operations of one moduli set fused into

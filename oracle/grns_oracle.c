@@ -875,6 +875,40 @@ long orc_max_eval_array(const orc_ctx *c, const int *x, size_t ld, size_t count,
     return bi;
 }
 
+/* mpint_div, src/mpint.cuh:286-296 (CPU path): rns_div_fast of the magnitudes, the Euclidean +1
+ * for a negative dividend with a nonzero remainder (int % on each digit), rns_eval_compute of the
+ * digits, sign 0 for a zero quotient.  Returns -1 where the division loop would not end. */
+int orc_mpint_div(const orc_ctx *c, int *rd, int *rs, orc_er_t *re, const int *xd, int xs, const int *yd, int ys) {
+    int n = c->n;
+    int rem[ORC_MAX_MODULI];
+    if (orc_div_fast(c, rd, rem, xd, yd) < 0) return -1;
+    int rz = 1;
+    for (int i = 0; i < n; i++) rz &= rem[i] == 0;
+    if (xs == 1 && !rz)
+        for (int i = 0; i < n; i++) rd[i] = (rd[i] + 1) % c->moduli[i];
+    orc_interval_t ev;
+    orc_eval(c, &ev, rd);
+    re[0] = ev.low;
+    re[1] = ev.upp;
+    int qz = 1;
+    for (int i = 0; i < n; i++) qz &= rd[i] == 0;
+    *rs = qz ? 0 : (xs ^ ys);
+    return 0;
+}
+
+void orc_mpint_div_array(const orc_ctx *c, size_t ld, size_t count, const int *xd, const int *xs, const int *yd,
+                         const int *ys, int *rd, int *rs, orc_er_t *re, int nthreads) {
+    long long cnt = (long long)count;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(orc_threads(nthreads))
+    for (long long j = 0; j < cnt; j++) {
+        int a[ORC_MAX_MODULI], b[ORC_MAX_MODULI], r[ORC_MAX_MODULI];
+        orc_gather(c, a, xd, ld, (size_t)j);
+        orc_gather(c, b, yd, ld, (size_t)j);
+        orc_mpint_div(c, r, &rs[j], &re[2 * j], a, xs[j], b, ys[j]);
+        orc_scatter(c, rd, r, ld, (size_t)j);
+    }
+}
+
 void orc_mpint_array(const orc_ctx *c, int op, size_t ld, size_t count,
                      const int *xd, const int *xs, const orc_er_t *xe,
                      const int *yd, const int *ys, const orc_er_t *ye,

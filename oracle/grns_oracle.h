@@ -122,6 +122,10 @@ void orc_scale2pow_array(const orc_ctx *c, const int *x, const unsigned int *D, 
 long orc_max_array(const orc_ctx *c, const int *x, size_t ld, size_t count, int *out_residues, int nthreads);
 /* The reference's rns_max pipeline (evaluate, then reduce with rns_max_cmp) on the CPU path. */
 long orc_max_eval_array(const orc_ctx *c, const int *x, size_t ld, size_t count, int nthreads);
+/* mpint_div (src/mpint.cuh:286-296, CPU path); -1 where the division loop would not end. */
+int orc_mpint_div(const orc_ctx *c, int *rd, int *rs, orc_er_t *re, const int *xd, int xs, const int *yd, int ys);
+void orc_mpint_div_array(const orc_ctx *c, size_t ld, size_t count, const int *xd, const int *xs, const int *yd,
+                         const int *ys, int *rd, int *rs, orc_er_t *re, int nthreads);
 /* mpint elementwise op over SoA arrays: op 0 add, 1 sub, 2 mul. Digits modulus-major with ld. */
 void orc_mpint_array(const orc_ctx *c, int op, size_t ld, size_t count,
                      const int *xd, const int *xs, const orc_er_t *xe,

@@ -129,6 +129,7 @@ def lib():
         L.orc_max_eval_array.restype = C.c_long
         L.orc_max_eval_array.argtypes = [CP, P, C.c_size_t, C.c_size_t, C.c_int]
         L.orc_mpint_array.argtypes = [CP, C.c_int, C.c_size_t, C.c_size_t, P, P, P, P, P, P, P, P, P, P, C.c_int]
+        L.orc_mpint_div_array.argtypes = [CP, C.c_size_t, C.c_size_t, P, P, P, P, P, P, P, C.c_int]
         _lib = L
     return _lib
 
@@ -263,6 +264,18 @@ class Oracle:
         x = _i32(x)
         cnt = x.shape[1]
         return lib().orc_max_eval_array(self._p, _ptr(x), cnt, cnt, nthreads)
+
+    def mpint_div_array(self, xd, xs, yd, ys, nthreads=0):
+        """mpint_div (Euclidean) of the magnitudes xd / yd with signs: (digits, sign, eval)."""
+        xd, yd = _i32(xd), _i32(yd)
+        xs, ys = _i32(xs), _i32(ys)
+        cnt = xd.shape[1]
+        rd = np.zeros_like(xd)
+        rs = np.zeros(cnt, np.int32)
+        re = np.zeros((cnt, 2), ER_DTYPE)
+        lib().orc_mpint_div_array(self._p, cnt, cnt, _ptr(xd), _ptr(xs), _ptr(yd), _ptr(ys), _ptr(rd), _ptr(rs),
+                                  _ptr(re), nthreads)
+        return rd, rs, re
 
     def mpint_array(self, op, xd, xs, xe, yd, ys, ye, nthreads=0):
         """op: 'add' | 'sub' | 'mul'.  xe/ye: ER_DTYPE arrays of shape (count, 2)."""

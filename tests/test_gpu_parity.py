@@ -471,3 +471,37 @@ def test_to_binary(ctxs, dev, n):
     # round trip through the words
     back = c.from_binary(c.to_binary(_dev(x, dev))).cpu().numpy()
     assert (back == x).all()
+
+
+@pytest.mark.parametrize("n", [4, 8, 32, 64])
+def test_mpint_div(ctxs, dev, n):
+    """mpint_div (Euclidean) against the CPU path: digits, sign and the eval bounds as bytes; the
+    digits are also those of the exact Euclidean quotient (up to the CPU path's m_i for 0)."""
+    c, o, mod = ctxs[n]
+    from grns_amd import product
+    M = product(mod)
+    L = M.bit_length() - 1
+    rnd = random.Random(400 + n)
+    cnt = {4: 1500, 8: 800, 32: 300, 64: 150}[n]
+    xs, ys = [], []
+    for _ in range(cnt):
+        xs.append(rnd.getrandbits(rnd.randrange(1, L)))
+        ys.append(max(1, rnd.getrandbits(rnd.randrange(1, L))))
+    xs += [0, 5, 7, mod[-1] * 3 + 1, mod[0] * mod[1] * 9 + 2, M - 1]
+    ys += [3, 7, 5, 3, 9, 1]
+    sx = [rnd.randrange(2) if v else 0 for v in xs]
+    sy = [rnd.randrange(2) for _ in ys]
+    xd, yd = to_res(xs, mod), to_res(ys, mod)
+    cnt = len(xs)
+    X = (_dev(xd, dev), _dev(np.array(sx, np.int32), dev), torch.zeros((cnt, 2, 2), dtype=torch.float64, device=dev))
+    Y = (_dev(yd, dev), _dev(np.array(sy, np.int32), dev), torch.zeros((cnt, 2, 2), dtype=torch.float64, device=dev))
+    Z = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
+    c.mpint("div", X, Y, Z, cnt)
+    rd, rs, re = o.mpint_div_array(xd, np.array(sx, np.int32), yd, np.array(sy, np.int32))
+    assert (Z[0].cpu().numpy() == rd).all()
+    assert (Z[1].cpu().numpy() == rs).all()
+    assert Z[2].cpu().numpy().tobytes() == re.tobytes()
+    m = np.array(mod, np.int64)[:, None]
+    q = [a // b + (1 if s and a % b else 0) for a, b, s in zip(xs, ys, sx)]
+    assert ((Z[0].cpu().numpy().astype(np.int64) % m) == to_res(q, mod)).all()
+    assert c.status() == 0
