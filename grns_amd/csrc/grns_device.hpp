@@ -753,7 +753,7 @@ GRNS_D unsigned group_words(const DevCtx *c, const Group &g, int xi) {
     const unsigned *tw = c->crt_words + g.gl;
     unsigned long long lo = 0, hi = 0;
     for (int i = 0; i < g.L; i++) {
-        const unsigned si = (unsigned)bcast_i(s, g.base + i);
+        const unsigned si = gword<GD>(g, (unsigned)s, i);
         const unsigned long long p = (unsigned long long)si * (wl ? tw[(size_t)i * WW] : 0u);
         lo += (unsigned)p;
         hi += p >> 32;
@@ -821,6 +821,7 @@ GRNS_D void group_div_words(const DevCtx *c, const Group &g, int xi, int di, int
     }
     const unsigned long long v1 = gword<GD>(g, D, wd - 1);
     const unsigned long long v2 = wd >= 2 ? gword<GD>(g, D, wd - 2) : 0u;
+    const double rv1 = 1.0 / (double)v1;  // the quotient estimate is corrected to the exact floor below
     const int top = wx - wd;
     unsigned Ds = wshift_up<GD>(g, D, top);
     unsigned Q = 0;
@@ -833,7 +834,7 @@ GRNS_D void group_div_words(const DevCtx *c, const Group &g, int xi, int di, int
             qh = 0xffffffffull;
             rh = num - qh * v1;
         } else {
-            qh = (unsigned long long)((double)num / (double)v1);
+            qh = (unsigned long long)(((double)u2 * 0x1p32 + (double)u1) * rv1);
             if (qh > 0xffffffffull) qh = 0xffffffffull;
             long long rr = (long long)(num - qh * v1);
             while (rr < 0) { qh--; rr += (long long)v1; }
