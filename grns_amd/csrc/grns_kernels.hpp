@@ -31,6 +31,16 @@ GRNS_D Group make_group(int G, int L) {
     return g;
 }
 
+// Block index in an order that gives each XCD a contiguous range: the hardware hands consecutive
+// workgroups to the 8 XCDs in turn, and the group kernels read residue rows whose cache lines hold
+// the numbers of neighbouring blocks, so neighbours should share an XCD (and its L2).  Needs a
+// grid that is a multiple of 8 (else the plain order).
+GRNS_D unsigned xcd_block() {
+    const unsigned b = blockIdx.x, nb = gridDim.x;
+    if (nb & 7u) return b;
+    return (b & 7u) * (nb >> 3) + (b >> 3);
+}
+
 // The k-th set bit of mask (k >= 0), or -1.
 GRNS_D int kth_bit(unsigned long long mask, int k) {
     for (int i = 0; i < k && mask; i++) mask &= mask - 1;
@@ -1143,7 +1153,7 @@ __global__ __launch_bounds__(BLOCK) void k_div_words(const DevCtx *__restrict__ 
                                                      RareQ rq) {
     const Group g = make_group(GD, c->n);
     constexpr int P = 64 / GD;
-    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    const long long wave = ((long long)xcd_block() * BLOCK + threadIdx.x) >> 6;
     const long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
     for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
         const long long j = j0 + (threadIdx.x & 63) / GD;
@@ -1238,7 +1248,7 @@ __global__ __launch_bounds__(BLOCK) void k_from_binary(const DevCtx *__restrict_
                                                        int *__restrict__ x, long long smod, long long snum) {
     const Group g = make_group(GD, c->n);
     constexpr int P = 64 / GD;
-    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    const long long wave = ((long long)xcd_block() * BLOCK + threadIdx.x) >> 6;
     const long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
     const bool act = g.gl < g.L;
     const int m = act ? c->m[g.gl] : 1;
@@ -1273,7 +1283,7 @@ __global__ __launch_bounds__(BLOCK) void k_to_binary(const DevCtx *__restrict__ 
                                                      unsigned *__restrict__ w, long long wst, long long wsn) {
     const Group g = make_group(GD, c->n);
     constexpr int P = 64 / GD;
-    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    const long long wave = ((long long)xcd_block() * BLOCK + threadIdx.x) >> 6;
     const long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
     const bool act = g.gl < g.L;
     const int m = act ? c->m[g.gl] : 1;
@@ -1377,7 +1387,7 @@ __global__ __launch_bounds__(BLOCK) void k_scale2pow_words(const DevCtx *__restr
                                                            int *__restrict__ y, RareQ rq) {
     const Group g = make_group(GD, c->n);
     constexpr int P = 64 / GD;
-    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    const long long wave = ((long long)xcd_block() * BLOCK + threadIdx.x) >> 6;
     const long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
     for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
         const long long j = j0 + (threadIdx.x & 63) / GD;
@@ -1606,7 +1616,7 @@ __global__ __launch_bounds__(BLOCK) void k_mpint_div(const DevCtx *__restrict__ 
                                                      int *__restrict__ status) {
     const Group g = make_group(GD, c->n);
     constexpr int P = 64 / GD;
-    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    const long long wave = ((long long)xcd_block() * BLOCK + threadIdx.x) >> 6;
     const long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
     int st = 0;
     for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {

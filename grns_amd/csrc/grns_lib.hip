@@ -204,6 +204,7 @@ long long grid_groups(size_t count, int G) {
     long long b = (waves + (BLOCK / 64) - 1) / (BLOCK / 64);
     if (b < 1) b = 1;
     if (b > (1ll << 20)) b = 1ll << 20;
+    if (b >= 8) b = (b + 7) / 8 * 8;  // a multiple of the XCD count (see xcd_block)
     return b;
 }
 
