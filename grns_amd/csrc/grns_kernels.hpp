@@ -81,6 +81,7 @@ struct EvalLeaf {
 
 // Common path of rns_eval_compute[_fast] for one number held by this lane.  Returns false when
 // the number needs mixed-radix conversion or refinement (then `out` is not final).
+// The scan of all residues before the trees also makes the compiler issue every load up front.
 template <int N, bool FAST>
 GRNS_HD bool eval_hot(const DevCtx *c, const int (&xr)[N], interval_t &out) {
     int nz = 0;
