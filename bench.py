@@ -179,6 +179,54 @@ def traffic_from_counters(fetch_dir, write_dir, out_path):
     log(f"[bench] wrote {out_path}")
 
 
+def cpu_per_op(threads, budget=2.0):
+    """The oracle (the reference's CPU path restated) on bounded samples of the eval/cmp (n = 32),
+    division/scaling (n = 64) and mpint mul (n = 32) workloads, each repeated for about `budget`
+    seconds on `threads` OpenMP threads: operations per second."""
+    import numpy as np
+    import torch
+
+    import grns_amd as G
+    from grns_amd.workload import Synth
+    from oracle.oracle import ER_DTYPE, Oracle
+
+    def rate(fn, count):
+        passes, t0 = 0, time.perf_counter()
+        while True:
+            fn()
+            passes += 1
+            dt = time.perf_counter() - t0
+            if dt >= budget:
+                return passes * count / dt, passes, dt
+
+    out = {}
+    mod32, mod64 = G.largest_primes_below(32), G.largest_primes_below(64)
+    o32, o64 = Oracle(mod32), Oracle(mod64)
+    s32 = Synth(mod32)
+    xe, ye, _ = s32.eval_cmp(1 << 18, seed=1)
+    xh, yh = xe.cpu().numpy(), ye.cpu().numpy()
+    for name, fn in (("eval", lambda: o32.eval_array(xh, nthreads=threads)),
+                     ("cmp", lambda: o32.cmp_array(xh, yh, nthreads=threads))):
+        v, p, dt = rate(fn, xh.shape[1])
+        out[name] = {"value": round(v), "unit": "ops/s", "cores": threads, "sample": f"2^18 of the n=32 workload x {p}, {dt:.1f}s"}
+    x, d, D, _ = Synth(mod64).div(1 << 12, seed=3, max_bits=1980, max_shift=1900)
+    xh, dh, Dh = x.cpu().numpy(), d.cpu().numpy(), D.cpu().numpy().view(np.uint32)
+    for name, fn in (("div_fast", lambda: o64.div_array(xh, dh, fast=True, nthreads=threads)),
+                     ("scale2pow", lambda: o64.scale2pow_array(xh, Dh, nthreads=threads))):
+        v, p, dt = rate(fn, xh.shape[1])
+        out[name] = {"value": round(v), "unit": "ops/s", "cores": threads, "sample": f"2^12 of the n=64 workload x {p}, {dt:.1f}s"}
+    c32 = G.Context(mod32, 0)
+    bits = torch.full((1 << 18,), 480, dtype=torch.int32, device="cuda")
+    X = [s32.bits(bits, 5, k, top_set=False) for k in (1, 2)]
+    E = [np.ascontiguousarray(c32.eval(t).cpu().numpy()).view(ER_DTYPE).reshape(-1, 2) for t in X]
+    sg = np.zeros(1 << 18, np.int32)
+    xa, ya = X[0].cpu().numpy(), X[1].cpu().numpy()
+    v, p, dt = rate(lambda: o32.mpint_array("mul", xa, sg, E[0], ya, sg, E[1], nthreads=threads), 1 << 18)
+    out["mpint_mul"] = {"value": round(v), "unit": "ops/s", "cores": threads,
+                        "sample": f"2^18 of the n=32 workload x {p}, {dt:.1f}s"}
+    return out
+
+
 def div_scale_leg(steps, warmup, n_mod=64, count=1 << 20, seed=3):
     """General division and power-of-two scaling: 64 largest primes below 2^31 (~1983-bit range),
     2^20 pairs, dividends uniform in [0, M), divisors of uniform bit length in [1, 1980] with
@@ -588,6 +636,8 @@ def main():
                          f"{sample.bit_length() - 1} numbers of the same workload, {passes} passes, "
                          f"{threads} OpenMP threads, {dt:.2f}s"}
         log(f"[bench] cpu baseline {cpu['value']} ops/s on {threads} threads")
+        extras["cpu_per_op"] = cpu_per_op(threads)
+        log(f"[bench] cpu per op: " + ", ".join(f"{k} {v['value']:.3g}/s" for k, v in extras["cpu_per_op"].items()))
 
     value = N / (ms_max * 1e-3)
     rl = roofline(max_bytes, kms_max, "k_max_partial<32, 1>")
