@@ -338,7 +338,7 @@ def batched_mpint_leg(steps, warmup, vectors=4096, max_len=65536, seed=4):
     return out
 
 
-def serving_requests(n_req, seed):
+def serving_requests(n_req, seed, lo=10, hi=22):
     """The serving workload's requests: each a dict with its context, batch tasks, a per-call
     closure, served and per-call outputs, and its inputs (kept alive until the run ends)."""
     import numpy as np
@@ -352,7 +352,7 @@ def serving_requests(n_req, seed):
     ctx = {k: G.Context(m, 0) for k, m in sets.items()}
     syn = {k: Synth(m) for k, m in sets.items()}
     kinds = rng.choice(["evalcmp", "max", "div", "scale", "mpint"], size=n_req, p=[0.3, 0.2, 0.2, 0.1, 0.2])
-    sizes = np.floor(np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 22), size=n_req))).astype(np.int64)
+    sizes = np.floor(np.exp(rng.uniform(np.log(2 ** lo), np.log(2 ** hi), size=n_req))).astype(np.int64)
     which = rng.choice([8, 32, 64], size=n_req)
     reqs = []
     elements = 0
@@ -417,13 +417,14 @@ def serving_requests(n_req, seed):
                 c.mpint("mul", X, Y, o, cnt)
             keep = (X, Y)
         elements += cnt
-        reqs.append({"ctx": c, "tasks": tasks, "plain": plain, "out_s": o_s, "out_p": o_p, "keep": keep,
+        reqs.append({"ctx": c, "tasks": G.Server.task_array(tasks), "plain": plain, "out_s": o_s, "out_p": o_p,
+                     "keep": keep,
                      "kind": kind, "n": n, "count": cnt})
     torch.cuda.synchronize()
     return reqs, elements, ctx
 
 
-def serving_leg(n_req=120, seed=7, budget=8 << 30):
+def serving_leg(n_req=120, seed=7, budget=8 << 30, lo=10, hi=22, max_batch=16):
     """Serving stream: requests with Poisson arrivals over three resident moduli
     sets (8, 32, 64 largest primes below 2^31), op mix 30% eval+cmp, 20% arg-max, 20% div_fast,
     10% scale2pow, 20% mpint mul, sizes log-uniform in [2^10, 2^22], operands drawn like the
@@ -436,7 +437,7 @@ def serving_leg(n_req=120, seed=7, budget=8 << 30):
 
     import grns_amd as G
 
-    reqs, elements, _ctx = serving_requests(n_req, seed)
+    reqs, elements, _ctx = serving_requests(n_req, seed, lo, hi)
     rng = np.random.default_rng(seed + 1)
 
     # plain per-call path: one request after another on the current stream
@@ -455,7 +456,7 @@ def serving_leg(n_req=120, seed=7, budget=8 << 30):
             for o in q["out_s"]:
                 o.zero_()
         torch.cuda.synchronize()
-        S = G.Server(0, hbm_budget=budget, slots=4, max_batch=16)
+        S = G.Server(0, hbm_budget=budget, slots=4, max_batch=max_batch)
         gaps = rng.exponential(1.0 / rate, size=n_req)
         tickets = []
         t0 = time.perf_counter()
@@ -482,7 +483,7 @@ def serving_leg(n_req=120, seed=7, budget=8 << 30):
     lam = 0.5 * cap
     t_lat, st_lat = served(lam)  # offered load half the per-call capacity: latency
     return {"workload": f"{n_req} requests, Poisson arrivals, mix 30% eval+cmp / 20% max / 20% div_fast / "
-                        f"10% scale2pow / 20% mpint mul, sizes log-uniform [2^10, 2^22], moduli sets of 8/32/64 "
+                        f"10% scale2pow / 20% mpint mul, sizes log-uniform [2^{lo}, 2^{hi}], moduli sets of 8/32/64 "
                         f"primes, {elements} elements",
             "elements": elements, "per_call_s": round(t_plain, 4), "per_call_elem_per_s": round(elements / t_plain),
             "saturated": {"offered_req_per_s": round(16 * cap, 1), "served_s": round(t_sat, 4),
@@ -609,10 +610,15 @@ def main():
 
     if "serve" in only or not args.only:
         extras["serving"] = serving_leg()
+        extras["serving_small"] = serving_leg(n_req=1000, seed=8, lo=4, hi=14, max_batch=64)
         sv = extras["serving"]
         log(f"[bench] serving: saturated {sv['saturated']['served_s']} s vs per-call {sv['per_call_s']} s "
             f"({sv['saturated']['speedup']}x); half load p50 {sv['half_load']['p50_ms']} ms, "
             f"p99 {sv['half_load']['p99_ms']} ms")
+        ss = extras["serving_small"]
+        log(f"[bench] serving small requests: saturated {ss['saturated']['served_s']} s vs per-call {ss['per_call_s']} s "
+            f"({ss['saturated']['speedup']}x); half load p50 {ss['half_load']['p50_ms']} ms, "
+            f"p99 {ss['half_load']['p99_ms']} ms")
         torch.cuda.empty_cache()
 
     # ------------------------------------------------------------------ CPU baseline

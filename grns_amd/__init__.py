@@ -486,10 +486,16 @@ class Server:
         _check(L.grns_serve_create(device, C.byref(cfg), C.byref(h)), "grns_serve_create")
         self._h = h
 
+    @staticmethod
+    def task_array(tasks):
+        """The ctypes task table of a request (build it once to resubmit a request cheaply)."""
+        return (BatchTask * len(tasks))(*tasks)
+
     def submit(self, ctx: Context, tasks):
-        arr = (BatchTask * len(tasks))(*tasks)
+        """Queue a request (a list of make_task() tasks or a task_array()); returns its ticket."""
+        arr = tasks if isinstance(tasks, C.Array) else self.task_array(tasks)
         t = C.c_longlong()
-        rc = lib().grns_serve_submit(self._h, ctx.handle, arr, len(tasks), C.byref(t))
+        rc = lib().grns_serve_submit(self._h, ctx.handle, arr, len(arr), C.byref(t))
         if rc != GRNS_OK:
             raise GrnsError(rc, "grns_serve_submit")
         return t.value

@@ -225,6 +225,7 @@ struct TaskPlan {
     std::vector<BatchTask> all;
     int nthr = 0, ngrp = 0;
     long long thr_tiles = 0, grp_tiles = 0;
+    long long grp_tile = 1;  // numbers per tile of the group list: a group does far more work per number
 };
 
 void make_plan(const grns_batch_task *u, int n, int ctx_n, const grns_batch_config &cfg, TaskPlan &P) {
@@ -232,6 +233,7 @@ void make_plan(const grns_batch_task *u, int n, int ctx_n, const grns_batch_conf
     std::vector<BatchTask> grp;
     const bool thread_ok = thread_kernel_available(ctx_n) && cfg.threads_per_number == 1;
     const long long tile = cfg.tile;
+    P.grp_tile = tile / 16 > 0 ? tile / 16 : 1;
     P.thr_tiles = P.grp_tiles = 0;
     for (int i = 0; i < n; i++) {
         if (u[i].count == 0 || u[i].op == GRNS_OP_MAX) continue;
@@ -242,7 +244,7 @@ void make_plan(const grns_batch_task *u, int n, int ctx_n, const grns_batch_conf
             P.all.push_back(t);
         } else {
             t.tile0 = P.grp_tiles;
-            P.grp_tiles += (t.count + tile - 1) / tile;
+            P.grp_tiles += (t.count + P.grp_tile - 1) / P.grp_tile;
             grp.push_back(t);
         }
     }
@@ -277,7 +279,7 @@ grns_status enqueue_plan(grns_ctx *c, const TaskPlan &P, const BatchTask *d_task
         // group width: the word division's when it has one (the group tasks are mostly divisions)
         const int G = words_group(c) ? words_group(c) : groups_for(c->n);
         hipLaunchKernelGGL(k_batch_group, grid, blk, 0, s, c->d, G, d_tasks + P.nthr, P.ngrp,
-                           P.grp_tiles, cfg.tile, d_counters + 1, c->d_status);
+                           P.grp_tiles, (int)P.grp_tile, d_counters + 1, c->d_status);
     }
     return hipGetLastError() == hipSuccess ? GRNS_OK : GRNS_ERR_HIP;
 }
