@@ -373,6 +373,28 @@ def test_mpint_known_answers(n):
         rd, rs, re, ovf = o.mpint_array(op, *X, *Y)
         vals, _ = _signed(o, rd, rs)
         assert vals == [want] and ovf[0] == 0
+    # mpint_div is Euclidean (divE in the same test): -55 = 13 * (-5) + 10
+    rd, rs, re = o.mpint_div_array(X[0], X[1], Y[0], Y[1])
+    vals, _ = _signed(o, rd, rs)
+    assert vals == [-5]
+
+
+def test_mpint_div_euclidean(orc):
+    """mpint_div over signed values: the Euclidean quotient (remainder in [0, |y|)), sign 0 for 0."""
+    mod = orc.moduli
+    M = product(mod)
+    rnd = random.Random(17)
+    bits = max(2, M.bit_length() - 2)
+    xs = [rnd.choice([-1, 1]) * rnd.randrange(1 << rnd.randrange(1, bits)) for _ in range(400)] + [0, -1, 1]
+    ys = [rnd.choice([-1, 1]) * max(1, rnd.randrange(1 << rnd.randrange(1, bits))) for _ in range(400)] + [5, 7, -7]
+    X, Y = _mpint_in(orc, xs), _mpint_in(orc, ys)
+    rd, rs, re = orc.mpint_div_array(X[0], X[1], Y[0], Y[1])
+    vals, _ = _signed(orc, rd, rs)
+    for a, b, v, s in zip(xs, ys, vals, rs):
+        q = abs(a) // abs(b) + (1 if a < 0 and abs(a) % abs(b) else 0)
+        want = -q if (a < 0) != (b < 0) else q
+        assert v == want, (a, b)
+        assert s == (1 if want < 0 else 0), (a, b)
 
 
 def test_mpint_random(orc):
