@@ -71,7 +71,7 @@ EXPORTS = [
     "grns_batch_create", "grns_batch_set_config", "grns_batch_get_config", "grns_batch_tune", "grns_batch_launch",
     "grns_batch_destroy", "grns_serve_create", "grns_serve_submit", "grns_serve_poll", "grns_serve_wait",
     "grns_serve_drain", "grns_serve_stats_get", "grns_serve_destroy", "grns_ctx_words", "grns_from_binary",
-    "grns_to_binary",
+    "grns_to_binary", "grns_mrc",
 ]
 DEBUG_EXPORTS = [
     "grns_debug_eval_hot", "grns_debug_filter", "grns_debug_cmp_lane", "grns_debug_max_key", "grns_debug_ceil_log2",
@@ -107,6 +107,7 @@ def lib():
     L.grns_scale2pow.argtypes = [P, P, LL, LL, P, SZ, P, P]
     L.grns_max_workspace_size.restype = SZ
     L.grns_ctx_words.argtypes = [P]
+    L.grns_mrc.argtypes = [P, P, LL, LL, SZ, P, LL, LL, P]
     L.grns_from_binary.argtypes = [P, P, LL, LL, I, P, SZ, P, LL, LL, P]
     L.grns_to_binary.argtypes = [P, P, LL, LL, SZ, P, LL, LL, P]
     L.grns_max_workspace_size.argtypes = [P, SZ]
@@ -286,6 +287,16 @@ class Context:
         _check(lib().grns_scale2pow(self._h, _tptr(x), sm, sn, _tptr(D), x.shape[1], _tptr(y),
                                     _stream_ptr(stream)), "grns_scale2pow")
         return y
+
+    def mrc(self, x, out=None, stream=None):
+        """Mixed-radix digits (n, count) of x (mrc, src/mrc.cuh:31-44)."""
+        import torch
+        out = torch.empty_like(x) if out is None else out
+        sm, sn = self._strides(x)
+        dm, dn = self._strides(out)
+        _check(lib().grns_mrc(self._h, _tptr(x), sm, sn, x.shape[1], _tptr(out), dm, dn, _stream_ptr(stream)),
+               "grns_mrc")
+        return out
 
     @property
     def words(self) -> int:

@@ -1307,6 +1307,26 @@ __global__ __launch_bounds__(BLOCK) void k_to_binary(const DevCtx *__restrict__ 
     }
 }
 
+// ---------------------------------------------------------------- mixed-radix digits
+// mrc (src/mrc.cuh:31-44): the digits of each number, digit k in group lane k (pipelined over
+// the lanes, table loads batched 8 steps at a time).
+static __global__ __launch_bounds__(BLOCK) void k_mrc(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
+                                               long long smod, long long snum, long long count,
+                                               int *__restrict__ d, long long dsm, long long dsn) {
+    const Group g = make_group(G, c->n);
+    const int P = 64 / G;
+    const long long wave = ((long long)xcd_block() * BLOCK + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
+    for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
+        const long long j = j0 + (threadIdx.x & 63) / G;
+        if (j >= count) continue;
+        const bool act = g.gl < g.L;
+        const int xi = act ? x[j * snum + (long long)g.gl * smod] : 0;
+        const int digit = group_mrc<8>(c, g, xi);
+        if (act) d[j * dsn + (long long)g.gl * dsm] = digit;
+    }
+}
+
 // ---------------------------------------------------------------- scaling by 2^D (group per number)
 // rns_rank_compute / _fast, src/rnsscal.cuh:17-62
 GRNS_D int group_rank(const DevCtx *c, const Group &g, int xi, int s) {

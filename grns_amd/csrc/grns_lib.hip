@@ -555,6 +555,19 @@ grns_status grns_div(grns_ctx *c, const int *x, const int *d, long long smod, lo
     return launch_check();
 }
 
+grns_status grns_mrc(grns_ctx *c, const int *x, long long smod, long long snum, size_t count, int *d, long long dsm,
+                     long long dsn, void *stream) {
+    if (c && c->device < 0) return GRNS_ERR_ARG;
+    if (!c || (count && (!x || !d))) return GRNS_ERR_ARG;
+    if (count == 0) return GRNS_OK;
+    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
+    DeviceGuard guard(c->device);
+    const int G = groups_for(c->n);
+    dim3 grid((unsigned)grid_groups(count, G)), blk(BLOCK);
+    hipLaunchKernelGGL(k_mrc, grid, blk, 0, S(stream), c->d, G, x, smod, snum, (long long)count, d, dsm, dsn);
+    return launch_check();
+}
+
 grns_status grns_from_binary(grns_ctx *c, const unsigned *w, long long wst, long long wsn, int nwords, const int *sign,
                              size_t count, int *x, long long smod, long long snum, void *stream) {
     if (c && c->device < 0) return GRNS_ERR_ARG;

@@ -84,6 +84,11 @@ grns_status grns_div(grns_ctx *ctx, const int *x, const int *d, long long mod_st
 grns_status grns_scale2pow(grns_ctx *ctx, const int *x, long long mod_stride, long long num_stride,
                            const unsigned *D, size_t count, int *y, void *stream);
 
+/* Mixed-radix digits (mrc, src/mrc.cuh:31-44; the reference's GPU forms src/mrc.cuh:101-181):
+ * digit k of number j at d[k * d_mod_stride + j * d_num_stride], x = sum_k d_k * m_0 ... m_{k-1}. */
+grns_status grns_mrc(grns_ctx *ctx, const int *x, long long mod_stride, long long num_stride, size_t count, int *d,
+                     long long d_mod_stride, long long d_num_stride, void *stream);
+
 /* Binary <-> RNS conversion.  Integers are little-endian 32-bit words: word k of number j at
  * w[j * num_stride + k * word_stride].
  * grns_from_binary: rns_from_binary (src/rnsbase.cuh:253-261, mpz_mod_ui per modulus): residue i

@@ -505,3 +505,30 @@ def test_mpint_div(ctxs, dev, n):
     q = [a // b + (1 if s and a % b else 0) for a, b, s in zip(xs, ys, sx)]
     assert ((Z[0].cpu().numpy().astype(np.int64) % m) == to_res(q, mod)).all()
     assert c.status() == 0
+
+
+@pytest.mark.parametrize("n", [4, 8, 32, 64])
+def test_mrc(ctxs, dev, n):
+    """Mixed-radix digits against the CPU path (also for non-canonical residues) and against the
+    integer they expand to."""
+    from grns_amd import product
+    c, o, mod = ctxs[n]
+    M = product(mod)
+    rnd = random.Random(500 + n)
+    vals = [rnd.randrange(M) for _ in range(400)] + [0, 1, M - 1, M // 2]
+    x = to_res(vals, mod)
+    for j in range(0, x.shape[1], 5):  # some residues at or above their modulus
+        i = rnd.randrange(len(mod))
+        if int(x[i, j]) + mod[i] < 2 ** 31:
+            x[i, j] += mod[i]
+    got = c.mrc(_dev(x, dev)).cpu().numpy()
+    for j in range(x.shape[1]):
+        assert got[:, j].tolist() == o.mrc(x[:, j]).tolist(), j
+    canon = to_res(vals, mod)
+    dg = c.mrc(_dev(canon, dev)).cpu().numpy()
+    for j, v in enumerate(vals):
+        acc, base = 0, 1
+        for k, m in enumerate(mod):
+            acc += int(dg[k, j]) * base
+            base *= m
+        assert acc == v
