@@ -147,9 +147,10 @@ GRNS_D void load_number(const int *__restrict__ x, long long smod, long long snu
 // algorithm to a second kernel: the streaming pass appends their indices here, and the rare pass
 // spreads them over every wave group of the device, so that a few long evaluations (refinement
 // loops, mixed-radix conversions) neither stall the streaming waves nor run one after another in
-// the wave that met them.  ctr[0] counts pushes, ctr[1] counts finished rare-pass blocks (the last
-// one clears the counters for the next call on the stream), ctr[2] is set when the queue was full;
-// then the overflowing items carry a marker in their output and the rare pass scans for it.
+// the wave that met them.  ctr[0] counts pushes and ctr[2] is set when the queue was full; then the
+// overflowing items carry a marker in their output and the rare pass scans for it.  A one-wave
+// kernel after the rare pass clears the counters for the next call on the stream.  Evaluation,
+// comparison, division and scaling share the queue of a (context, stream) pair, one call at a time.
 struct RareQ {
     long long *items;
     unsigned *ctr;
@@ -1216,15 +1217,14 @@ template <bool FAST>
 __global__ __launch_bounds__(BLOCK) void k_div(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
                                                const int *__restrict__ d, long long smod, long long snum,
                                                long long count, int *__restrict__ q, int *__restrict__ r,
-                                               const long long *__restrict__ order, int *__restrict__ status) {
+                                               int *__restrict__ status) {
     Group g = make_group(G, c->n);
     const int P = 64 / G;
     long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
     long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
     for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
-        long long jj = j0 + (threadIdx.x & 63) / G;
-        if (jj < count) {
-            long long j = order ? order[jj] : jj;
+        long long j = j0 + (threadIdx.x & 63) / G;
+        if (j < count) {
             long long off = j * snum + (long long)g.gl * smod;
             int xi = g.gl < g.L ? x[off] : 0;
             int di = g.gl < g.L ? d[off] : 0;
@@ -1452,16 +1452,14 @@ static __global__ __launch_bounds__(BLOCK) void k_scale2pow_rare(const DevCtx *_
 
 static __global__ __launch_bounds__(BLOCK) void k_scale2pow(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,
                                                      long long smod, long long snum, const unsigned *__restrict__ D,
-                                                     long long count, int *__restrict__ y,
-                                                     const long long *__restrict__ order) {
+                                                     long long count, int *__restrict__ y) {
     Group g = make_group(G, c->n);
     const int P = 64 / G;
     long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
     long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
     for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
-        long long jj = j0 + (threadIdx.x & 63) / G;
-        if (jj < count) {
-            long long j = order ? order[jj] : jj;
+        long long j = j0 + (threadIdx.x & 63) / G;
+        if (j < count) {
             long long off = j * snum + (long long)g.gl * smod;
             int xi = g.gl < g.L ? x[off] : 0;
             int v = group_scale2pow(c, g, xi, D[j]);

@@ -550,8 +550,8 @@ grns_status grns_div(grns_ctx *c, const int *x, const int *d, long long smod, lo
         hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), rq.ctr);
         return launch_check();
     }
-    if (fast) hipLaunchKernelGGL((k_div<true>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, (const long long *)nullptr, c->d_status);
-    else hipLaunchKernelGGL((k_div<false>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, (const long long *)nullptr, c->d_status);
+    if (fast) hipLaunchKernelGGL((k_div<true>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, c->d_status);
+    else hipLaunchKernelGGL((k_div<false>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, c->d_status);
     return launch_check();
 }
 
@@ -642,8 +642,7 @@ grns_status grns_scale2pow(grns_ctx *c, const int *x, long long smod, long long 
         hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), rq.ctr);
         return launch_check();
     }
-    hipLaunchKernelGGL(k_scale2pow, grid, blk, 0, S(stream), c->d, G, x, smod, snum, D, cnt, y,
-                       (const long long *)nullptr);
+    hipLaunchKernelGGL(k_scale2pow, grid, blk, 0, S(stream), c->d, G, x, smod, snum, D, cnt, y);
     return launch_check();
 }
 
