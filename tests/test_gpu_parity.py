@@ -532,3 +532,20 @@ def test_mrc(ctxs, dev, n):
             acc += int(dg[k, j]) * base
             base *= m
         assert acc == v
+
+
+@pytest.mark.parametrize("fast", [False, True])
+def test_eval_cmp_small_config_full_size(ctxs, dev, fast):
+    """The CPU-runnable configuration at full size: 8 largest primes below 2^31, N = 65536 numbers
+    uniform in [0, M) with 1% forced to 0, M-1, M/2 and M/2 +- 1, pairs (x, x + delta) with delta in
+    {-1, 0, +1, uniform}: every interval (as bytes) and every sign equal the CPU path's."""
+    from grns_amd.workload import Synth
+    c, o, mod = ctxs[8]
+    x, y, forced = Synth(mod).eval_cmp(65536, seed=11)
+    assert len(forced) == 655
+    xh, yh = x.cpu().numpy(), y.cpu().numpy()
+    ev = c.eval(x, fast=fast).cpu().numpy()
+    assert ev.tobytes() == o.eval_array(xh, fast=fast).tobytes()
+    if not fast:
+        assert (c.cmp(x, y).cpu().numpy() == o.cmp_array(xh, yh)).all()
+    assert c.status() == 0
