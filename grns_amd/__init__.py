@@ -67,7 +67,7 @@ _lib = None
 EXPORTS = [
     "grns_version", "grns_ctx_create", "grns_ctx_destroy", "grns_ctx_n", "grns_ctx_log2m", "grns_ctx_constants",
     "grns_ctx_tables", "grns_ctx_status", "grns_eval", "grns_cmp", "grns_div", "grns_scale2pow",
-    "grns_max_workspace_size", "grns_max", "grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul", "grns_mpint_div",
+    "grns_max_workspace_size", "grns_max", "grns_max_mrc", "grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul", "grns_mpint_div",
     "grns_batch_create", "grns_batch_set_config", "grns_batch_get_config", "grns_batch_tune", "grns_batch_launch",
     "grns_batch_destroy", "grns_serve_create", "grns_serve_submit", "grns_serve_poll", "grns_serve_wait",
     "grns_serve_drain", "grns_serve_stats_get", "grns_serve_destroy", "grns_ctx_words", "grns_from_binary",
@@ -112,6 +112,7 @@ def lib():
     L.grns_to_binary.argtypes = [P, P, LL, LL, SZ, P, LL, LL, P]
     L.grns_max_workspace_size.argtypes = [P, SZ]
     L.grns_max.argtypes = [P, P, LL, LL, SZ, P, P, P, SZ, P]
+    L.grns_max_mrc.argtypes = [P, P, LL, LL, SZ, P, P, P, SZ, P]
     for nm in ("grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul", "grns_mpint_div"):
         getattr(L, nm).argtypes = [P, C.POINTER(MpintArray), C.POINTER(MpintArray), C.POINTER(MpintArray), SZ, P, P]
     L.grns_batch_create.argtypes = [P, C.POINTER(BatchTask), I, C.POINTER(P)]
@@ -328,7 +329,9 @@ class Context:
     def max_workspace_size(self, count) -> int:
         return lib().grns_max_workspace_size(self._h, count)
 
-    def max(self, x, workspace=None, out=None, out_idx=None, stream=None):
+    def max(self, x, workspace=None, out=None, out_idx=None, stream=None, mrc=False):
+        """Arg-max (residues, index) of the numbers x (n, count); mrc=True runs the mixed-radix
+        pipeline (grns_max_mrc), which returns the same result."""
         import torch
         cnt = x.shape[1]
         if workspace is None:
@@ -338,8 +341,9 @@ class Context:
         if out_idx is None:
             out_idx = torch.empty(1, dtype=torch.int64, device=x.device)
         sm, sn = self._strides(x)
-        _check(lib().grns_max(self._h, _tptr(x), sm, sn, cnt, _tptr(out), _tptr(out_idx), _tptr(workspace),
-                              workspace.numel(), _stream_ptr(stream)), "grns_max")
+        fn, name = (lib().grns_max_mrc, "grns_max_mrc") if mrc else (lib().grns_max, "grns_max")
+        _check(fn(self._h, _tptr(x), sm, sn, cnt, _tptr(out), _tptr(out_idx), _tptr(workspace), workspace.numel(),
+                  _stream_ptr(stream)), name)
         return out, out_idx
 
     def mpint(self, op, x, y, z, count, overflow=None, stream=None):

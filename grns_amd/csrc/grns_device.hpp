@@ -43,6 +43,21 @@ GRNS_HD const T *reload_point(const T *p) {
 #endif
 }
 
+// reload_point placed after the computation of `dep` (a per-lane value): the loads through the
+// returned pointer cannot be issued before `dep` is known, which bounds how far ahead of their use
+// the compiler issues constant loads in long unrolled sequences.
+template <class T>
+GRNS_HD const T *reload_after(const T *p, int dep) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    long long v = reinterpret_cast<long long>(p);
+    asm volatile("" : "+s"(v) : "v"(dep));
+    return reinterpret_cast<const T *>(v);
+#else
+    (void)dep;
+    return p;
+#endif
+}
+
 // A per-lane 32-bit value the optimiser may not widen or fold: a byte offset that stays a 32-bit
 // vector offset of scalar-base global loads.
 GRNS_HD unsigned lane_offset(unsigned v) {
@@ -126,6 +141,7 @@ struct DevCtx {
     int pow2_inv[THRESH][MAXN];              // RNS_POW2_INVERSE
     const int *pow2;                         // RNS_POW2, (log2M+1) x n, device memory
     const int *mrc_inv;                      // MRC_MULT_INV, n x n, device memory
+    const double *mrc_inv_d;                 // the same as doubles (per-lane mixed-radix conversion)
     int words;                               // 32-bit words of sum_i s_i (M / m_i) < n M
     const unsigned *crt_words;               // (n + 1) x words: M / m_i for i < n, then M
 };

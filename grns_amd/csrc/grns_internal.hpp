@@ -15,6 +15,7 @@ struct grns_ctx {
     grns::DevCtx *d = nullptr;     // device constants
     int *d_pow2 = nullptr;
     int *d_mrc = nullptr;
+    double *d_mrc_d = nullptr;
     int *d_status = nullptr;
     unsigned *d_crt = nullptr;      // DevCtx::crt_words
     int max_blocks = 2048;         // arg-max stage-1 grid: one resident wave of blocks on the device
@@ -41,6 +42,7 @@ struct grns_ctx {
     StageTimer timers[3];
     std::vector<int> pow2;
     std::vector<int> mrc;
+    std::vector<double> mrc_d;
     std::vector<unsigned> crt;
 };
 

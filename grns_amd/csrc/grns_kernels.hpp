@@ -10,6 +10,8 @@
 //    (mixed-radix conversion, refinement), which the wave resolves cooperatively after a ballot.
 #pragma once
 
+#include <utility>
+
 #include "grns_device.hpp"
 
 namespace grns {
@@ -70,7 +72,14 @@ struct EvalLeaf {
             ru = const_view(cc->ru + I);
         }
         constexpr int k = I % 4;
-        double s = mulmod_recip_d_nowrap((double)xr[I], dw[k], dm[k], dinv[k]);
+        int xi = xr[I];
+#if defined(__HIP_DEVICE_COMPILE__)
+        // Keeps the conversion after the loads of the other residues: left free, the scheduler
+        // moves the first one up to its load, and the wave then waits for that load before it
+        // issues the rest.
+        asm volatile("" : "+v"(xi));
+#endif
+        double s = mulmod_recip_d_nowrap((double)xi, dw[k], dm[k], dinv[k]);
         // s in [1, 2^31): the reference's int result is s itself and both products are at least
         // 2^-32, so their bounds (and those of every partial sum) are the neighbouring doubles
         smax = fmax(smax, fabs(s - 0x1p30));
@@ -1083,6 +1092,198 @@ __global__ __launch_bounds__(FBLOCK) void k_max_final(const DevCtx *__restrict__
     }
     if (T < N && win >= 0) out[T] = x[win * snum + (long long)T * smod];
     if (T == 0) *out_idx = win;
+}
+
+// ---------------------------------------------------------------- arg-max by mixed-radix digits
+// The mixed-radix pipeline of tests/test_perf_rnsmax.cu:95-156 (rns_max_mrc): the digits of every
+// number, then the largest digit vector, compared from the most significant digit down (mrs_cmp,
+// src/mrc.cuh:13-22).  Here one lane converts one number in registers and keeps its best digit
+// vector in a column of LDS; the value of a number is that of its residues taken mod m_i, and
+// equal values go to the lower index, so the result is the one grns_max returns.
+
+// Mixed-radix digits of the canonical number held by this lane (mrc, src/mrc.cuh:31-44): step j
+// turns residue i > j into (x_i - d_j) m_j^-1 mod m_i.  Each product is reduced exactly in FP64:
+// with d in (-m_i, m_i) and the inverse in [0, m_i), hi + lo = d * inv exactly (|d * inv| < 2^62);
+// floor(hi / m_i - 2^-19), computed as one fma with RN(1/m_i), is floor(d * inv / m_i) or one less
+// (its error is below 2^-20), so r = d * inv - q m_i is an exact integer in [0, 2 m_i) and one
+// conditional subtraction (an unsigned minimum) ends the reduction.
+// One chunk of step J: residues I0 .. I0 + 3 (those below N).  The constants of the chunk are
+// loaded (scalar loads) once the chunk before the previous one is computed (d2): one chunk of
+// look-ahead rather than the whole table in scalar registers.
+template <int N, int J, int I0>
+GRNS_D void lane_mrc_chunk(const DevCtx *c, const double *tab, int (&mr)[N], int &d1, int &d2) {
+    const DevCtx *cc = reload_after(c, d2);
+    cdouble *inv = const_view(reload_after(tab, d2) + J * N + I0);
+    cdouble *dm = const_view(cc->dm + I0);
+    cdouble *dinv = const_view(cc->dinv + I0);
+    cint *mi = const_view(cc->m + I0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (I0 + k < N) {
+            const double d = (double)(mr[I0 + k] - mr[J]);
+            const double hi = d * inv[k];
+            const double lo = fma(d, inv[k], -hi);
+            const double q = floor(fma(hi, dinv[k], -0x1p-19));
+            const unsigned r = (unsigned)(fma(-q, dm[k], hi) + lo);
+            const unsigned s = r - (unsigned)mi[k];
+            mr[I0 + k] = (int)(s < r ? s : r);
+        }
+    }
+    d2 = d1;
+    d1 = mr[I0 + 3 < N ? I0 + 3 : N - 1];
+}
+template <int N, int J, int... C>
+GRNS_D void lane_mrc_step(const DevCtx *c, const double *tab, int (&mr)[N], int &d1, int &d2,
+                          std::integer_sequence<int, C...>) {
+    (lane_mrc_chunk<N, J, J + 1 + 4 * C>(c, tab, mr, d1, d2), ...);
+}
+template <int N, int... J>
+GRNS_D void lane_mrc_steps(const DevCtx *c, const double *tab, int (&mr)[N], std::integer_sequence<int, J...>) {
+    int d1 = 0, d2 = 0;
+    (lane_mrc_step<N, J>(c, tab, mr, d1, d2, std::make_integer_sequence<int, (N - 1 - J + 3) / 4>{}), ...);
+}
+// Every step, as straight-line code (all register indexes are constants).  tab: the context's
+// inverse table as doubles (c->mrc_inv_d), read once by the caller.
+template <int N>
+GRNS_D void lane_mrc(const DevCtx *c, const double *tab, int (&mr)[N]) {
+    lane_mrc_steps<N>(c, tab, mr, std::make_integer_sequence<int, N - 1>{});
+}
+
+// Order of the digit vector `a` (this lane's registers) against column `col` of `bd` below the top
+// digit, which the caller found equal: 1 above, -1 below, 0 equal.
+template <int N>
+GRNS_D int mrc_cmp_lower(const int (&a)[N], const int (*bd)[BLOCK], int col) {
+    int r = 0;
+#pragma unroll
+    for (int k = N - 2; k >= 0; k--) {
+        const int b = bd[k][col];
+        if (r == 0 && a[k] != b) r = a[k] > b ? 1 : -1;
+    }
+    return r;
+}
+
+// Order of columns a and b of `bd` (mrs_cmp).
+template <int N>
+GRNS_D int mrc_cmp_cols(const int (*bd)[BLOCK], int a, int b) {
+#pragma unroll 1
+    for (int k = N - 1; k >= 0; k--) {
+        const int u = bd[k][a], v = bd[k][b];
+        if (u != v) return u > v ? 1 : -1;
+    }
+    return 0;
+}
+
+// Stage 1: each block reduces a contiguous chunk to one candidate (index and digits).  Lanes walk
+// their numbers in increasing index order, so a lane keeps its incumbent on equal values.
+template <int N, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_max_mrc_partial(const DevCtx *__restrict__ c, const int *__restrict__ x,
+                                                           long long smod, long long snum, long long count,
+                                                           long long chunk, long long *__restrict__ part_idx,
+                                                           int *__restrict__ part_dig) {
+    __shared__ int bd[N][BLOCK];
+    __shared__ long long sidx[BLOCK];
+    const int t = threadIdx.x;
+    const long long start = (long long)blockIdx.x * chunk;
+    const long long end = start + chunk < count ? start + chunk : count;
+    const double *tab = c->mrc_inv_d;
+    long long best = -1;
+    int top = 0;
+    for (long long b0 = start; b0 < end; b0 += BLOCK) {
+        const long long j = b0 + t;
+        if (j >= end) continue;
+        int xr[N];
+        load_number<N, MODE>(x, smod, snum, j, xr);
+        bool canon = true;
+        cint *cm = const_view(reload_point(c)->m);
+#pragma unroll
+        for (int i = 0; i < N; i++) canon &= (unsigned)xr[i] < (unsigned)cm[i];
+        if (!canon) {  // residues taken mod m_i (constants through views: no addresses hoisted)
+#pragma unroll
+            for (int i0 = 0; i0 < N; i0 += 8) {
+                const DevCtx *cc = reload_point(c);
+                cint *m8 = const_view(cc->m + i0);
+                cdouble *v8 = const_view(cc->dinv + i0);
+#pragma unroll
+                for (int k = 0; k < 8 && i0 + k < N; k++) xr[i0 + k] = canon_res(xr[i0 + k], m8[k], v8[k]);
+            }
+        }
+        lane_mrc<N>(c, tab, xr);
+        int r = 1;
+        if (best >= 0) {
+            r = (xr[N - 1] > top) - (xr[N - 1] < top);
+            if (r == 0) r = mrc_cmp_lower<N>(xr, bd, t);
+        }
+        if (r > 0) {
+            best = j;
+            top = xr[N - 1];
+#pragma unroll
+            for (int k = 0; k < N; k++) bd[k][t] = xr[k];
+        }
+    }
+    sidx[t] = best;
+    __syncthreads();
+    for (int h = BLOCK / 2; h >= 1; h >>= 1) {
+        if (t < h) {
+            const long long a = sidx[t], b = sidx[t + h];
+            bool take = false;
+            if (b >= 0) {
+                if (a < 0) {
+                    take = true;
+                } else {
+                    const int r = mrc_cmp_cols<N>(bd, t + h, t);
+                    take = r > 0 || (r == 0 && b < a);
+                }
+            }
+            if (take) {
+#pragma unroll 1
+                for (int k = 0; k < N; k++) bd[k][t] = bd[k][t + h];
+                sidx[t] = b;
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) part_idx[blockIdx.x] = sidx[0];
+    if (t < N) part_dig[(long long)blockIdx.x * N + t] = bd[t][0];
+}
+
+// Whether candidate a of stage 1 comes before candidate b: the larger digit vector, then the
+// lower index.
+template <int N>
+GRNS_D bool mrc_part_before(const long long *__restrict__ part_idx, const int *__restrict__ part_dig, int a, int b) {
+    const int *da = part_dig + (long long)a * N;
+    const int *db = part_dig + (long long)b * N;
+#pragma unroll 1
+    for (int k = N - 1; k >= 0; k--)
+        if (da[k] != db[k]) return da[k] > db[k];
+    return part_idx[a] < part_idx[b];
+}
+
+// Stage 2 (one block): the best of the stage-1 candidates; the winner's residues are copied out.
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_max_mrc_final(const int *__restrict__ x, long long smod, long long snum,
+                                                         const long long *__restrict__ part_idx,
+                                                         const int *__restrict__ part_dig, int nparts,
+                                                         int *__restrict__ out, long long *__restrict__ out_idx) {
+    __shared__ int sk[BLOCK];
+    const int t = threadIdx.x;
+    int bk = -1;
+    for (int k = t; k < nparts; k += BLOCK) {
+        if (part_idx[k] < 0) continue;
+        if (bk < 0 || mrc_part_before<N>(part_idx, part_dig, k, bk)) bk = k;
+    }
+    sk[t] = bk;
+    __syncthreads();
+    for (int h = BLOCK / 2; h >= 1; h >>= 1) {
+        if (t < h) {
+            const int a = sk[t], b = sk[t + h];
+            if (b >= 0 && (a < 0 || mrc_part_before<N>(part_idx, part_dig, b, a))) sk[t] = b;
+        }
+        __syncthreads();
+    }
+    const int w = sk[0];
+    const long long win = w >= 0 ? part_idx[w] : -1;
+    if (t < N && win >= 0) out[t] = x[win * snum + (long long)t * smod];
+    if (t == 0) *out_idx = win;
 }
 
 // ---------------------------------------------------------------- division (group per number)

@@ -166,6 +166,7 @@ grns_status build_constants(grns_ctx *c, const int *moduli, int n) {
     c->mrc.assign((size_t)n * n, 0);
     for (int i = 0; i < n; i++)
         for (int j = i + 1; j < n; j++) c->mrc[(size_t)i * n + j] = inverse_mod(moduli[i], moduli[j]);
+    c->mrc_d.assign(c->mrc.begin(), c->mrc.end());
     // M / m_i and M as 32-bit words, one word more than M has (room for sum_i s_i M / m_i < n M)
     const int WW = (int)limbs.size() + 1;
     h.words = WW;
@@ -336,16 +337,19 @@ grns_status grns_ctx_create(const int *moduli, int n, int device, grns_ctx **out
         hipMalloc(&c->d_crt, sizeof(unsigned) * c->crt.size()) != hipSuccess ||
         hipMalloc(&c->d_pow2, sizeof(int) * c->pow2.size()) != hipSuccess ||
         hipMalloc(&c->d_mrc, sizeof(int) * c->mrc.size()) != hipSuccess ||
+        hipMalloc(&c->d_mrc_d, sizeof(double) * c->mrc_d.size()) != hipSuccess ||
         hipMalloc(&c->d_status, sizeof(int)) != hipSuccess) {
         grns_ctx_destroy(c);
         return GRNS_ERR_OOM;
     }
     c->h.pow2 = c->d_pow2;
     c->h.mrc_inv = c->d_mrc;
+    c->h.mrc_inv_d = c->d_mrc_d;
     c->h.crt_words = c->d_crt;
     if (hipMemcpy(c->d_crt, c->crt.data(), sizeof(unsigned) * c->crt.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d_pow2, c->pow2.data(), sizeof(int) * c->pow2.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d_mrc, c->mrc.data(), sizeof(int) * c->mrc.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_mrc_d, c->mrc_d.data(), sizeof(double) * c->mrc_d.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d, &c->h, sizeof(DevCtx), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->d_status, 0, sizeof(int)) != hipSuccess) {
         grns_ctx_destroy(c);
@@ -364,6 +368,7 @@ grns_status grns_ctx_destroy(grns_ctx *c) {
         if (c->d_pow2) (void)hipFree(c->d_pow2);
         if (c->d_crt) (void)hipFree(c->d_crt);
         if (c->d_mrc) (void)hipFree(c->d_mrc);
+        if (c->d_mrc_d) (void)hipFree(c->d_mrc_d);
         if (c->d_status) (void)hipFree(c->d_status);
         for (auto &q : c->queues) {
             (void)hipFree(q.items);
@@ -711,6 +716,44 @@ grns_status grns_max(grns_ctx *c, const int *x, long long smod, long long snum, 
     return launch_check();
 }
 
+grns_status grns_max_mrc(grns_ctx *c, const int *x, long long smod, long long snum, size_t count, int *out_residues,
+                         long long *out_idx, void *workspace, size_t ws_bytes, void *stream) {
+    if (c && c->device < 0) return GRNS_ERR_ARG;
+    if (!c || !x || !out_residues || !out_idx || count == 0) return GRNS_ERR_ARG;
+    long long nb, ch;
+    max_shape(c, count, nb, ch);
+    // the same workspace as grns_max: nb indices, then nb digit vectors (fewer bytes than nb
+    // candidates with their residue copies)
+    if (!workspace || ws_bytes < max_ws_bytes(c, nb)) return GRNS_ERR_ARG;
+    DeviceGuard guard(c->device);
+    long long *pidx = reinterpret_cast<long long *>(workspace);
+    int *pdig = reinterpret_cast<int *>(pidx + nb);
+    const bool mm = snum == 1 && count <= ((size_t)1 << 29);
+    dim3 grid((unsigned)nb), blk(BLOCK);
+    long long cnt = (long long)count;
+#define GRNS_MAX_MRC_CASE(NN)                                                                                   \
+    case NN:                                                                                                    \
+        if (mm) hipLaunchKernelGGL((k_max_mrc_partial<NN, LD_MM>), grid, blk, 0, S(stream), c->d, x, smod, snum, \
+                                   cnt, ch, pidx, pdig);                                                        \
+        else hipLaunchKernelGGL((k_max_mrc_partial<NN, LD_ANY>), grid, blk, 0, S(stream), c->d, x, smod, snum,   \
+                                cnt, ch, pidx, pdig);                                                           \
+        if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                               \
+        hipLaunchKernelGGL((k_max_mrc_final<NN>), dim3(1), blk, 0, S(stream), x, smod, snum, pidx, pdig, (int)nb, \
+                           out_residues, out_idx);                                                              \
+        break;
+    switch (c->n) {
+        GRNS_MAX_MRC_CASE(4)
+        GRNS_MAX_MRC_CASE(8)
+        GRNS_MAX_MRC_CASE(16)
+        GRNS_MAX_MRC_CASE(32)
+        GRNS_MAX_MRC_CASE(64)
+        default:
+            return GRNS_ERR_UNSUPPORTED;
+    }
+#undef GRNS_MAX_MRC_CASE
+    return launch_check();
+}
+
 static grns_status mpint_launch(grns_ctx *c, int op, const grns_mpint_array *x, const grns_mpint_array *y,
                                 const grns_mpint_array *z, size_t count, unsigned char *ovf, void *stream) {
     if (!c || !x || !y || !z || c->device < 0) return GRNS_ERR_ARG;
@@ -819,6 +862,7 @@ DevCtx host_view(const grns_ctx *c) {
     DevCtx h = c->h;
     h.pow2 = c->pow2.data();
     h.mrc_inv = c->mrc.data();
+    h.mrc_inv_d = c->mrc_d.data();
     h.crt_words = c->crt.data();
     return h;
 }

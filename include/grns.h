@@ -111,6 +111,15 @@ size_t grns_max_workspace_size(const grns_ctx *ctx, size_t count);
 grns_status grns_max(grns_ctx *ctx, const int *x, long long mod_stride, long long num_stride, size_t count,
                      int *out_residues, long long *out_idx, void *workspace, size_t workspace_bytes, void *stream);
 
+/* Arg-max by mixed-radix digits: rns_max_mrc (tests/test_perf_rnsmax.cu:145-153, with
+ * rns_max_mrc_compute_kernel and rns_max_mrc_tree_kernel :109-138).  Every number is converted to
+ * mixed-radix digits (src/mrc.cuh:31-44) and the largest digit vector wins (mrs_cmp,
+ * src/mrc.cuh:13-22).  Same arguments, workspace and results as grns_max: residues are taken mod
+ * m_i, equal maxima give the lowest index. */
+grns_status grns_max_mrc(grns_ctx *ctx, const int *x, long long mod_stride, long long num_stride, size_t count,
+                         int *out_residues, long long *out_idx, void *workspace, size_t workspace_bytes,
+                         void *stream);
+
 /* mp_int_t (src/types.cuh:45-49) as arrays: digits modulus-major (digits[i * ld + j]),
  * sign[count], eval[2 * count] (eval[2j] lower, eval[2j+1] upper bound). */
 typedef struct {

@@ -283,9 +283,13 @@ def test_scale2pow_parity(ctxs, dev, n):
     assert (y.cpu().numpy() == ref).all()
 
 
+@pytest.mark.parametrize("mrc", [False, True])
 @pytest.mark.parametrize("n", [4, 8, 32, 64])
 @pytest.mark.parametrize("case", ["random", "planted", "equal", "small"])
-def test_max_parity(ctxs, dev, n, case):
+def test_max_parity(ctxs, dev, n, case, mrc):
+    """Arg-max by keys + exact order (grns_max) and by mixed-radix digits (grns_max_mrc): the
+    index of the largest value (lowest among equal ones) and its residues, against the integers
+    and the CPU path."""
     c, o, mod = ctxs[n]
     from grns_amd.moduli import product
     M = product(mod)
@@ -304,11 +308,56 @@ def test_max_parity(ctxs, dev, n, case):
             for k in idx[1:]:
                 vals[k] = V - 1
     x = to_res(vals, mod)
-    res, idx = c.max(_dev(x, dev))
+    res, idx = c.max(_dev(x, dev), mrc=mrc)
     ridx, rres = o.max_array(x)
     truth = max(range(len(vals)), key=lambda k: (vals[k], -k))
     assert int(idx.item()) == truth == ridx
     assert (res.cpu().numpy() == rres).all()
+
+
+@pytest.mark.parametrize("n", [8, 32, 64])
+def test_max_noncanonical_residues(ctxs, dev, n):
+    """Residues stand for their value mod m_i (some raised by m_i, some made negative): both
+    arg-max pipelines return the index of the largest canonical value, lowest among equal ones,
+    and that number's residues as given."""
+    c, o, mod = ctxs[n]
+    from grns_amd.moduli import product
+    M = product(mod)
+    rnd = random.Random(70 + n)
+    vals = [rnd.randrange(M) for _ in range(6000)]
+    top = max(vals)
+    for k in rnd.sample(range(len(vals)), 5):  # ties at the maximum, the first of them wins
+        vals[k] = top
+    x = to_res(vals, mod).astype(np.int64)
+    for j in range(0, x.shape[1], 3):
+        i = rnd.randrange(len(mod))
+        x[i, j] += mod[i] if x[i, j] + mod[i] < 2 ** 31 else -mod[i]
+    for j in range(1, x.shape[1], 7):
+        i = rnd.randrange(len(mod))
+        x[i, j] -= mod[i]  # negative residue standing for the same value
+    x = x.astype(np.int32)
+    truth = max(range(len(vals)), key=lambda k: (vals[k], -k))
+    for mrc in (False, True):
+        res, idx = c.max(_dev(x, dev), mrc=mrc)
+        assert int(idx.item()) == truth, mrc
+        assert res.cpu().numpy().tolist() == x[:, truth].tolist(), mrc
+
+
+@pytest.mark.parametrize("n", [8, 32])
+def test_max_mrc_number_major(ctxs, dev, n):
+    """Number-major input (residue i of number j at x[j * n + i]): the strided path of the mixed-
+    radix arg-max equals the modulus-major one."""
+    c, o, mod = ctxs[n]
+    from grns_amd.moduli import product
+    M = product(mod)
+    rnd = random.Random(90 + n)
+    vals = [rnd.randrange(M) for _ in range(5000)]
+    x = to_res(vals, mod)
+    xt = _dev(np.ascontiguousarray(x.T), dev).t()  # (n, count) view with strides (1, n)
+    res, idx = c.max(xt, mrc=True)
+    truth = max(range(len(vals)), key=lambda k: (vals[k], -k))
+    assert int(idx.item()) == truth
+    assert res.cpu().numpy().tolist() == x[:, truth].tolist()
 
 
 def _mpint_operands(mod, o, count, seed, bits):
@@ -553,12 +602,16 @@ def test_eval_cmp_small_config_full_size(ctxs, dev, fast):
 
 def test_max_headline_config_full_size(ctxs, dev):
     """The arg-max configuration at full size: 32 largest primes below 2^31, N = 2^24 numbers in
-    [0, M/2), a planted unique maximum and 1000 near-ties one below it: index and residues equal
-    the planted ones and the CPU path's evaluate-then-reduce arg-max."""
+    [0, M/2), a planted unique maximum and 1000 near-ties one below it: index and residues of both
+    arg-max pipelines (keys, mixed-radix digits) equal the planted ones and the CPU path's
+    evaluate-then-reduce arg-max."""
     from grns_amd.workload import Synth
     c, o, mod = ctxs[32]
     x, p, V = Synth(mod).max_planted(1 << 24, seed=2)
     res, idx = c.max(x)
+    assert int(idx.item()) == p
+    assert res.cpu().numpy().tolist() == [V % m for m in mod]
+    res, idx = c.max(x, mrc=True)
     assert int(idx.item()) == p
     assert res.cpu().numpy().tolist() == [V % m for m in mod]
     assert o.max_eval_array(x.cpu().numpy()) == p
