@@ -551,6 +551,16 @@ def main():
     max_bytes = 4 * n * N
     kms_max = kernel_ms(ctx, "max", lambda: ctx.max(x, workspace=ws, out=out, out_idx=oidx), args.steps)
     log(f"[bench] max: {ms_max:.4f} ms/step, stage-1 kernel {kms_max:.4f} ms")
+    if "max" in only:
+        # the mixed-radix pipeline on the same numbers (VALU-bound: n(n-1)/2 exact FP64 modular
+        # products per number); its result is checked like the headline's
+        ctx.max(x, workspace=ws, out=out, out_idx=oidx, mrc=True)
+        if int(oidx.item()) != p or out.cpu().numpy().tolist() != residues_of(V, mod):
+            raise SystemExit("mixed-radix arg-max check failed")
+        ms_mrc = timed(lambda: ctx.max(x, workspace=ws, out=out, out_idx=oidx, mrc=True), args.steps, args.warmup)
+        extras["max_mrc"] = {"ms_per_step": round(ms_mrc, 4), "ops_per_s": round(N / (ms_mrc * 1e-3)),
+                             "bound": "valu", "digit_products_per_number": n * (n - 1) // 2}
+        log(f"[bench] max by mixed-radix digits: {ms_mrc:.4f} ms/step")
 
     # ------------------------------------------------------------------ eval / cmp
     if only & {"eval", "cmp"}:
