@@ -33,19 +33,31 @@ FLAGS = [
 ]
 
 
-def _compile(srcs, out, verbose):
-    tmp = out + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *[os.path.join(CSRC, s) for s in srcs]]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, out)
+def _run(cmds, verbose):
+    """Run the commands in parallel; raise if any fails."""
+    procs = []
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
+    failed = [cmd for cmd, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
 
 
 def build(verbose: bool = False) -> str:
-    """Compile libgrns_hip.so (the hot path) and libgrns_workload.so (synthetic inputs)."""
-    _compile(SOURCES, LIB, verbose)
-    _compile(["grns_workload.hip"], WORKLOAD_LIB, verbose)
+    """Compile libgrns_hip.so (the hot path) and libgrns_workload.so (synthetic inputs): every
+    source to an object in parallel, then one link per library."""
+    obj_dir = os.path.join(HERE, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    units = SOURCES + ["grns_workload.hip"]
+    objs = {u: os.path.join(obj_dir, u.replace(".hip", ".o")) for u in units}
+    _run([[HIPCC, *compile_flags, "-c", "-o", objs[u], os.path.join(CSRC, u)] for u in units], verbose)
+    links = [(LIB, [objs[u] for u in SOURCES]), (WORKLOAD_LIB, [objs["grns_workload.hip"]])]
+    _run([[HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp", *o] for out, o in links], verbose)
+    for out, _ in links:
+        os.replace(out + ".tmp", out)
     return LIB
 
 

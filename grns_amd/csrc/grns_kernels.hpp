@@ -1174,8 +1174,10 @@ GRNS_D int mrc_cmp_cols(const int (*bd)[BLOCK], int a, int b) {
 }
 
 // Stage 1: each block reduces a contiguous chunk to one candidate (index and digits).  Lanes walk
-// their numbers in increasing index order, so a lane keeps its incumbent on equal values.
-template <int N, int MODE>
+// their numbers in increasing index order, so a lane keeps its incumbent on equal values.  The
+// load form is chosen per number (LD_AUTO): the kernel is VALU-bound and one instance per n keeps
+// the straight-line conversion code compiled once.
+template <int N>
 __global__ __launch_bounds__(BLOCK) void k_max_mrc_partial(const DevCtx *__restrict__ c, const int *__restrict__ x,
                                                            long long smod, long long snum, long long count,
                                                            long long chunk, long long *__restrict__ part_idx,
@@ -1192,7 +1194,7 @@ __global__ __launch_bounds__(BLOCK) void k_max_mrc_partial(const DevCtx *__restr
         const long long j = b0 + t;
         if (j >= end) continue;
         int xr[N];
-        load_number<N, MODE>(x, smod, snum, j, xr);
+        load_number<N, LD_AUTO>(x, smod, snum, j, xr);
         bool canon = true;
         cint *cm = const_view(reload_point(c)->m);
 #pragma unroll
@@ -1525,6 +1527,61 @@ static __global__ __launch_bounds__(BLOCK) void k_mrc(const DevCtx *__restrict__
         const int xi = act ? x[j * snum + (long long)g.gl * smod] : 0;
         const int digit = group_mrc<8>(c, g, xi);
         if (act) d[j * dsn + (long long)g.gl * dsm] = digit;
+    }
+}
+
+// The numbers of the lanes in `mask` converted by the wave groups with the reference's steps
+// (group_mrc), which define the digits of residues outside [0, m_i).  Not inlined: its registers
+// stay out of the per-lane kernel.
+template <int N>
+__device__ __attribute__((noinline)) void coop_mrc_lanes(const DevCtx *c, const int *__restrict__ x, long long smod,
+                                                         long long snum, long long j, unsigned long long mask,
+                                                         int *__restrict__ d, long long dsm, long long dsn) {
+    const int G = Pow2Ceil<N>::v;
+    Group g = make_group(G, N);
+    const int P = 64 / G;
+    const int gid = (threadIdx.x & 63) / G;
+    while (mask) {
+        const int src = kth_bit(mask, gid);
+        const long long jl = __shfl(j, src < 0 ? 0 : src, 64);
+        mask = drop_low_bits(mask, P);
+        if (src >= 0) {
+            const bool act = g.gl < g.L;
+            const int xi = act ? x[jl * snum + (long long)g.gl * smod] : 0;
+            const int digit = group_mrc<8>(c, g, xi);
+            if (act) d[jl * dsn + (long long)g.gl * dsm] = digit;
+        }
+    }
+}
+
+// Batched mixed-radix conversion, one number per lane: canonical numbers convert in their lane
+// (lane_mrc, exact); a number with a residue outside [0, m_i) goes to the wave's groups.  Load form
+// per number (LD_AUTO), as in k_max_mrc_partial.
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_mrc_lane(const DevCtx *__restrict__ c, const int *__restrict__ x,
+                                                    long long smod, long long snum, long long count,
+                                                    int *__restrict__ d, long long dsm, long long dsn) {
+    const double *tab = c->mrc_inv_d;
+    for (long long base = (long long)blockIdx.x * BLOCK; base < count; base += (long long)gridDim.x * BLOCK) {
+        const long long j = base + threadIdx.x;
+        bool coop = false;
+        if (j < count) {
+            int xr[N];
+            load_number<N, LD_AUTO>(x, smod, snum, j, xr);
+            bool canon = true;
+            cint *cm = const_view(reload_point(c)->m);
+#pragma unroll
+            for (int i = 0; i < N; i++) canon &= (unsigned)xr[i] < (unsigned)cm[i];
+            if (canon) {
+                lane_mrc<N>(c, tab, xr);
+#pragma unroll
+                for (int i = 0; i < N; i++) d[j * dsn + (long long)i * dsm] = xr[i];
+            } else {
+                coop = true;
+            }
+        }
+        const unsigned long long mask = __ballot(coop);
+        if (mask) coop_mrc_lanes<N>(c, x, smod, snum, j, mask, d, dsm, dsn);
     }
 }
 

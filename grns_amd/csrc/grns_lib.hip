@@ -567,9 +567,25 @@ grns_status grns_mrc(grns_ctx *c, const int *x, long long smod, long long snum, 
     if (count == 0) return GRNS_OK;
     if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
     DeviceGuard guard(c->device);
+    const long long cnt = (long long)count;
+    dim3 blk(BLOCK), lgrid((unsigned)grid_elems(count));
+#define GRNS_MRC_CASE(NN)                                                                                         \
+    case NN:                                                                                                      \
+        hipLaunchKernelGGL((k_mrc_lane<NN>), lgrid, blk, 0, S(stream), c->d, x, smod, snum, cnt, d, dsm, dsn);     \
+        return launch_check();
+    switch (c->n) {
+        GRNS_MRC_CASE(4)
+        GRNS_MRC_CASE(8)
+        GRNS_MRC_CASE(16)
+        GRNS_MRC_CASE(32)
+        GRNS_MRC_CASE(64)
+        default:
+            break;
+    }
+#undef GRNS_MRC_CASE
     const int G = groups_for(c->n);
-    dim3 grid((unsigned)grid_groups(count, G)), blk(BLOCK);
-    hipLaunchKernelGGL(k_mrc, grid, blk, 0, S(stream), c->d, G, x, smod, snum, (long long)count, d, dsm, dsn);
+    dim3 grid((unsigned)grid_groups(count, G));
+    hipLaunchKernelGGL(k_mrc, grid, blk, 0, S(stream), c->d, G, x, smod, snum, cnt, d, dsm, dsn);
     return launch_check();
 }
 
@@ -728,15 +744,12 @@ grns_status grns_max_mrc(grns_ctx *c, const int *x, long long smod, long long sn
     DeviceGuard guard(c->device);
     long long *pidx = reinterpret_cast<long long *>(workspace);
     int *pdig = reinterpret_cast<int *>(pidx + nb);
-    const bool mm = snum == 1 && count <= ((size_t)1 << 29);
     dim3 grid((unsigned)nb), blk(BLOCK);
     long long cnt = (long long)count;
 #define GRNS_MAX_MRC_CASE(NN)                                                                                   \
     case NN:                                                                                                    \
-        if (mm) hipLaunchKernelGGL((k_max_mrc_partial<NN, LD_MM>), grid, blk, 0, S(stream), c->d, x, smod, snum, \
-                                   cnt, ch, pidx, pdig);                                                        \
-        else hipLaunchKernelGGL((k_max_mrc_partial<NN, LD_ANY>), grid, blk, 0, S(stream), c->d, x, smod, snum,   \
-                                cnt, ch, pidx, pdig);                                                           \
+        hipLaunchKernelGGL((k_max_mrc_partial<NN>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, ch, pidx,   \
+                           pdig);                                                                               \
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                               \
         hipLaunchKernelGGL((k_max_mrc_final<NN>), dim3(1), blk, 0, S(stream), x, smod, snum, pidx, pdig, (int)nb, \
                            out_residues, out_idx);                                                              \

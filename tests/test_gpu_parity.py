@@ -564,7 +564,7 @@ def test_mrc(ctxs, dev, n):
     c, o, mod = ctxs[n]
     M = product(mod)
     rnd = random.Random(500 + n)
-    vals = [rnd.randrange(M) for _ in range(400)] + [0, 1, M - 1, M // 2]
+    vals = [rnd.randrange(M) for _ in range(3000)] + [0, 1, M - 1, M // 2]
     x = to_res(vals, mod)
     for j in range(0, x.shape[1], 5):  # some residues at or above their modulus
         i = rnd.randrange(len(mod))
@@ -573,6 +573,8 @@ def test_mrc(ctxs, dev, n):
     got = c.mrc(_dev(x, dev)).cpu().numpy()
     for j in range(x.shape[1]):
         assert got[:, j].tolist() == o.mrc(x[:, j]).tolist(), j
+    xt = _dev(np.ascontiguousarray(x.T), dev).t()  # number-major input (and output)
+    assert (c.mrc(xt).cpu().numpy() == got).all()
     canon = to_res(vals, mod)
     dg = c.mrc(_dev(canon, dev)).cpu().numpy()
     for j, v in enumerate(vals):
@@ -615,3 +617,42 @@ def test_max_headline_config_full_size(ctxs, dev):
     assert int(idx.item()) == p
     assert res.cpu().numpy().tolist() == [V % m for m in mod]
     assert o.max_eval_array(x.cpu().numpy()) == p
+
+
+@pytest.mark.parametrize("n", [12, 20])
+def test_other_moduli_counts(dev, n):
+    """A moduli count with no per-lane kernel (12: groups of 16 lanes, 20: groups of 32 with idle
+    lanes) runs the group kernels: evaluation, comparison, mixed-radix digits, division and scaling
+    against the CPU path, the binary round trip against the integers."""
+    import grns_amd as G
+    from grns_amd import product
+    from oracle.oracle import Oracle
+    mod = G.largest_primes_below(n)
+    M = product(mod)
+    c, o = G.Context(mod, 0), Oracle(mod)
+    try:
+        vals = eval_inputs(mod, 1500, seed=40 + n)
+        x = to_res(vals, mod)
+        for fast in (False, True):
+            assert c.eval(_dev(x, dev), fast=fast).cpu().numpy().tobytes() == o.eval_array(x, fast=fast).tobytes()
+        xs, ys = cmp_pairs(mod, 1500, seed=41 + n)
+        xa, ya = to_res(xs, mod), to_res(ys, mod)
+        assert (c.cmp(_dev(xa, dev), _dev(ya, dev)).cpu().numpy() == o.cmp_array(xa, ya)).all()
+        dg = c.mrc(_dev(x, dev)).cpu().numpy()
+        for j in range(0, x.shape[1], 7):
+            assert dg[:, j].tolist() == o.mrc(x[:, j]).tolist(), j
+        dx, dd = div_inputs(mod, 200, seed=42 + n)
+        q, r = c.div(_dev(to_res(dx, mod), dev), _dev(to_res(dd, mod), dev), fast=True)
+        rq, rr = o.div_array(to_res(dx, mod), to_res(dd, mod), fast=True)
+        assert (q.cpu().numpy() == rq).all() and (r.cpu().numpy() == rr).all()
+        rnd = random.Random(43 + n)
+        D = np.array([rnd.randrange(0, M.bit_length() + 5) for _ in vals], np.uint32)
+        y = c.scale2pow(_dev(x, dev), _dev(D.view(np.int32), dev)).cpu().numpy()
+        assert (y == o.scale2pow_array(x, D)).all()
+        w = c.to_binary(_dev(x, dev)).cpu().numpy().view(np.uint32)
+        assert [sum(int(v) << (32 * k) for k, v in enumerate(row)) for row in w] == [v % M for v in vals]
+        assert (c.from_binary(c.to_binary(_dev(x, dev))).cpu().numpy() == x).all()
+        assert c.status() == 0
+    finally:
+        c.close()
+        o.close()
