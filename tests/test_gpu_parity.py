@@ -430,6 +430,27 @@ def test_rare_queue_overflow(ctxs, dev, n):
     assert c.status() == 0
 
 
+@pytest.mark.parametrize("n", [8, 32])
+def test_rare_items_distinct(ctxs, dev, n):
+    """2^17 + 999 distinct numbers just below M in shuffled order (each needs the rare pass of
+    evaluation and of comparison): every queued item is resolved for its own number; every
+    interval (as bytes) and sign equals the CPU path's."""
+    c, o, mod = ctxs[n]
+    from grns_amd.moduli import product
+    M = product(mod)
+    count = (1 << 17) + 999
+    rnd = random.Random(66 + n)
+    ks = list(range(count))
+    rnd.shuffle(ks)
+    xs = [M - 1 - k for k in ks]
+    ys = [v - 1 if i % 2 else v + (1 if v + 1 < M else -2) for i, v in enumerate(xs)]
+    xa, ya = to_res(xs, mod), to_res(ys, mod)
+    xd, yd = _dev(xa, dev), _dev(ya, dev)
+    assert _intervals_bytes(c.eval(xd)) == o.eval_array(xa).tobytes()
+    assert (c.cmp(xd, yd).cpu().numpy() == o.cmp_array(xa, ya)).all()
+    assert c.status() == 0
+
+
 def test_rare_queue_per_stream(ctxs, dev):
     """Calls on two streams use their own rare-item queues."""
     c, o, mod = ctxs[32]
