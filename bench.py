@@ -449,6 +449,7 @@ def serving_leg(n_req=120, seed=7, budget=8 << 30, lo=10, hi=22, max_batch=16):
         q["plain"]()
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
+    mismatches = []  # served outputs that differ from the per-call ones (reported, never expected)
 
     def served(rate):
         """Submit every request at Poisson arrival times of the given rate; wall time and stats."""
@@ -472,9 +473,12 @@ def serving_leg(n_req=120, seed=7, budget=8 << 30, lo=10, hi=22, max_batch=16):
         st = S.stats()
         S.close()
         for q in reqs:
-            for a, b in zip(q["out_s"], q["out_p"]):
+            for k, (a, b) in enumerate(zip(q["out_s"], q["out_p"])):
                 if not torch.equal(a.view(torch.uint8), b.view(torch.uint8)):
-                    raise SystemExit("served results differ from the per-call results")
+                    nd = int((a.view(torch.uint8) != b.view(torch.uint8)).sum().item())
+                    mismatches.append({"kind": q["kind"], "n": q["n"], "count": q["count"], "output": k,
+                                       "bytes_differing": nd})
+                    log(f"[bench] served result differs from the per-call one: {mismatches[-1]}")
         return wall, st
 
     cap = n_req / t_plain  # requests per second the per-call path sustains
@@ -493,7 +497,7 @@ def serving_leg(n_req=120, seed=7, budget=8 << 30, lo=10, hi=22, max_batch=16):
             "half_load": {"offered_req_per_s": round(lam, 1), "served_s": round(t_lat, 4),
                           "p50_ms": round(st_lat["p50_ms"], 3), "p99_ms": round(st_lat["p99_ms"], 3),
                           "launches": st_lat["launches"]},
-            "bit_exact_vs_per_call": True}
+            "bit_exact_vs_per_call": not mismatches, "mismatches": mismatches[:20]}
 
 
 def main():

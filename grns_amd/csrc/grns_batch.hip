@@ -34,12 +34,22 @@ struct BatchTask {
     MpintArgs mp;
 };
 
-GRNS_D int find_task(const BatchTask *__restrict__ tasks, int ntasks, long long t) {
-    int lo = 0, hi = ntasks - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (tasks[mid].tile0 <= t) lo = mid;
-        else hi = mid - 1;
+// The task holding tile t: the last task whose first tile is <= t (tasks[0].tile0 = 0 <= t and
+// the first tiles increase strictly: every task has a tile).  Wave 0 searches 64-ary: each lane
+// tests one candidate and a ballot keeps the last one at or below t, so 4096 tasks take two
+// rounds of dependent loads instead of twelve.
+GRNS_D int find_task_wave(const BatchTask *__restrict__ tasks, int ntasks, long long t) {
+    const int lane = threadIdx.x & 63;
+    int lo = 0, span = ntasks;
+    while (span > 1) {
+        const int step = (span + 63) / 64;
+        const int idx = lo + lane * step;
+        const bool le = lane * step < span && tasks[idx].tile0 <= t;
+        const unsigned long long m = __ballot(le);  // bit 0 set: tasks[lo].tile0 <= t
+        const int last = 63 - __clzll(m);
+        const int nlo = lo + last * step;
+        span = (lo + span - nlo) < step ? (lo + span - nlo) : step;
+        lo = nlo;
     }
     return lo;
 }
@@ -49,10 +59,15 @@ GRNS_D bool next_tile(const BatchTask *__restrict__ tasks, int ntasks, long long
                       long long &t, int &k) {
     __shared__ long long s_t;
     __shared__ int s_k;
-    if (threadIdx.x == 0) {
-        long long v = (long long)atomicAdd(counter, 1ull);
-        s_t = v;
-        s_k = v < total ? find_task(tasks, ntasks, v) : 0;
+    if (threadIdx.x < 64) {
+        unsigned long long v = 0;
+        if (threadIdx.x == 0) v = atomicAdd(counter, 1ull);
+        v = __shfl(v, 0, 64);
+        const int kk = (long long)v < total ? find_task_wave(tasks, ntasks, (long long)v) : 0;
+        if (threadIdx.x == 0) {
+            s_t = (long long)v;
+            s_k = kk;
+        }
     }
     __syncthreads();
     // the same for the whole block: keep them in scalar registers (task fields become scalar loads)

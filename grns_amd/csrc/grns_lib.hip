@@ -225,7 +225,9 @@ struct DeviceGuard {
 
 constexpr unsigned RARE_CAP = 1u << 20;  // queued rare items per stream (8 MiB of indices)
 
-// The rare-item queue of `stream` (allocated and cleared on the first call on that stream).
+// The rare-item queue of `stream` (allocated and cleared on the first call on that stream).  The
+// counters are cleared by a kernel on `stream` itself: a hipMemset is ordered on the null stream
+// only, and the kernels of a non-blocking stream could otherwise read the counters before it.
 grns_status rare_queue(grns_ctx *c, void *stream, RareQ &q) {
     std::lock_guard<std::mutex> lock(c->qmu);
     for (auto &e : c->queues)
@@ -241,7 +243,8 @@ grns_status rare_queue(grns_ctx *c, void *stream, RareQ &q) {
         (void)hipFree(e.items);
         return GRNS_ERR_OOM;
     }
-    if (hipMemset(e.ctr, 0, 4 * sizeof(unsigned)) != hipSuccess) {
+    hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), e.ctr);
+    if (hipGetLastError() != hipSuccess) {
         (void)hipFree(e.items);
         (void)hipFree(e.ctr);
         return GRNS_ERR_HIP;
@@ -351,7 +354,8 @@ grns_status grns_ctx_create(const int *moduli, int n, int device, grns_ctx **out
         hipMemcpy(c->d_mrc, c->mrc.data(), sizeof(int) * c->mrc.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d_mrc_d, c->mrc_d.data(), sizeof(double) * c->mrc_d.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d, &c->h, sizeof(DevCtx), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(c->d_status, 0, sizeof(int)) != hipSuccess) {
+        hipMemset(c->d_status, 0, sizeof(int)) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {  // tables and status in place before any stream's kernels
         grns_ctx_destroy(c);
         return GRNS_ERR_HIP;
     }
@@ -419,6 +423,7 @@ grns_status grns_ctx_status(grns_ctx *c, int *bits) {
     HIP_OK(hipDeviceSynchronize());
     HIP_OK(hipMemcpy(bits, c->d_status, sizeof(int), hipMemcpyDeviceToHost));
     HIP_OK(hipMemset(c->d_status, 0, sizeof(int)));
+    HIP_OK(hipDeviceSynchronize());  // cleared before any later kernel, on any stream, can raise a bit
     return GRNS_OK;
 }
 

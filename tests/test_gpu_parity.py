@@ -451,6 +451,36 @@ def test_rare_items_distinct(ctxs, dev, n):
     assert c.status() == 0
 
 
+def test_first_call_on_nonblocking_stream(dev):
+    """The first call of a fresh context on a fresh non-blocking stream (torch streams are) creates
+    that stream's rare-item queue; its counters must be clear before the call's kernels push to
+    it, or queued numbers are lost.  Scaling with non-canonical residues queues every such number."""
+    import grns_amd as G
+    from oracle.oracle import Oracle
+    mod = SETS[8]
+    o = Oracle(mod)
+    vals = eval_inputs(mod, 20000, seed=64)
+    x = to_res(vals, mod).astype(np.int64)
+    for j in range(0, x.shape[1], 3):  # one residue raised by its modulus: the rare path
+        if x[0, j] + mod[0] < 2 ** 31:
+            x[0, j] += mod[0]
+    x = x.astype(np.int32)
+    rnd = random.Random(65)
+    D = np.array([rnd.randrange(1, 200) for _ in vals], np.uint32)
+    ref = o.scale2pow_array(x, D)
+    xd, Dd = _dev(x, dev), _dev(D.view(np.int32), dev)
+    torch.cuda.synchronize()  # the inputs are in place before another stream reads them
+    for _ in range(4):
+        c = G.Context(mod, 0)
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            y = c.scale2pow(xd, Dd)
+        s.synchronize()
+        assert (y.cpu().numpy() == ref).all()
+        c.close()
+    o.close()
+
+
 def test_rare_queue_per_stream(ctxs, dev):
     """Calls on two streams use their own rare-item queues."""
     c, o, mod = ctxs[32]
