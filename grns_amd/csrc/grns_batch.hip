@@ -79,7 +79,11 @@ GRNS_D bool next_tile(const BatchTask *__restrict__ tasks, int ntasks, long long
     return t < total;
 }
 
-template <int N>
+// Op classes of a thread-list launch: the kernel holds only the code of the classes its tasks use,
+// so a list of mpint tasks runs with the registers (and occupancy) of the mpint code alone.
+enum : int { OPS_EVALCMP = 1, OPS_MPINT = 2 };
+
+template <int N, int OPS>
 __global__ __launch_bounds__(BLOCK) void k_batch_thread(const DevCtx *__restrict__ c, const BatchTask *__restrict__ tasks,
                                                         int ntasks, long long total, int tile,
                                                         unsigned long long *counter, int *__restrict__ status) {
@@ -91,27 +95,34 @@ __global__ __launch_bounds__(BLOCK) void k_batch_thread(const DevCtx *__restrict
         long long e1 = e0 + tile < T.count ? e0 + tile : T.count;
         switch (T.op) {
             case GRNS_OP_EVAL:
-            case GRNS_OP_EVAL_FAST: {
-                interval_t *o = reinterpret_cast<interval_t *>(T.o0);
-                for (long long base = e0; base < e1; base += BLOCK) {
-                    if (T.op == GRNS_OP_EVAL) eval_step<N, false>(c, T.a, T.asm_, T.asn, base + threadIdx.x, e1, o, status);
-                    else eval_step<N, true>(c, T.a, T.asm_, T.asn, base + threadIdx.x, e1, o, status);
+            case GRNS_OP_EVAL_FAST:
+                if constexpr ((OPS & OPS_EVALCMP) != 0) {
+                    interval_t *o = reinterpret_cast<interval_t *>(T.o0);
+                    for (long long base = e0; base < e1; base += BLOCK) {
+                        if (T.op == GRNS_OP_EVAL)
+                            eval_step<N, false>(c, T.a, T.asm_, T.asn, base + threadIdx.x, e1, o, status);
+                        else eval_step<N, true>(c, T.a, T.asm_, T.asn, base + threadIdx.x, e1, o, status);
+                    }
                 }
                 break;
-            }
             case GRNS_OP_CMP:
-                for (long long base = e0; base < e1; base += BLOCK)
-                    cmp_step<N>(c, T.a, T.asm_, T.asn, T.b, T.bsm, T.bsn, base + threadIdx.x, e1,
-                                reinterpret_cast<int *>(T.o0), status);
+                if constexpr ((OPS & OPS_EVALCMP) != 0) {
+                    for (long long base = e0; base < e1; base += BLOCK)
+                        cmp_step<N>(c, T.a, T.asm_, T.asn, T.b, T.bsm, T.bsn, base + threadIdx.x, e1,
+                                    reinterpret_cast<int *>(T.o0), status);
+                }
                 break;
             case GRNS_OP_MPINT_ADD:
-                for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 0>(c, T.mp, j);
+                if constexpr ((OPS & OPS_MPINT) != 0)
+                    for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 0>(c, T.mp, j);
                 break;
             case GRNS_OP_MPINT_SUB:
-                for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 1>(c, T.mp, j);
+                if constexpr ((OPS & OPS_MPINT) != 0)
+                    for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 1>(c, T.mp, j);
                 break;
             case GRNS_OP_MPINT_MUL:
-                for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 2>(c, T.mp, j);
+                if constexpr ((OPS & OPS_MPINT) != 0)
+                    for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 2>(c, T.mp, j);
                 break;
             default:
                 break;
@@ -241,6 +252,7 @@ struct TaskPlan {
     int nthr = 0, ngrp = 0;
     long long thr_tiles = 0, grp_tiles = 0;
     long long grp_tile = 1;  // numbers per tile of the group list: a group does far more work per number
+    int thr_ops = 0;         // op classes (OPS_*) of the thread list
 };
 
 void make_plan(const grns_batch_task *u, int n, int ctx_n, const grns_batch_config &cfg, TaskPlan &P) {
@@ -250,12 +262,16 @@ void make_plan(const grns_batch_task *u, int n, int ctx_n, const grns_batch_conf
     const long long tile = cfg.tile;
     P.grp_tile = tile / 16 > 0 ? tile / 16 : 1;
     P.thr_tiles = P.grp_tiles = 0;
+    P.thr_ops = 0;
     for (int i = 0; i < n; i++) {
         if (u[i].count == 0 || u[i].op == GRNS_OP_MAX) continue;
         BatchTask t = to_dev(u[i]);
         if (thread_ok && elementwise(u[i].op)) {
             t.tile0 = P.thr_tiles;
             P.thr_tiles += (t.count + tile - 1) / tile;
+            P.thr_ops |= (u[i].op == GRNS_OP_MPINT_ADD || u[i].op == GRNS_OP_MPINT_SUB || u[i].op == GRNS_OP_MPINT_MUL)
+                             ? OPS_MPINT
+                             : OPS_EVALCMP;
             P.all.push_back(t);
         } else {
             t.tile0 = P.grp_tiles;
@@ -277,8 +293,15 @@ grns_status enqueue_plan(grns_ctx *c, const TaskPlan &P, const BatchTask *d_task
         switch (c->n) {
 #define GRNS_BT(NN)                                                                                          \
     case NN:                                                                                                 \
-        hipLaunchKernelGGL((k_batch_thread<NN>), grid, blk, 0, s, c->d, d_tasks, P.nthr, P.thr_tiles, cfg.tile, \
-                           d_counters, c->d_status);                                                         \
+        if (P.thr_ops == OPS_MPINT)                                                                          \
+            hipLaunchKernelGGL((k_batch_thread<NN, OPS_MPINT>), grid, blk, 0, s, c->d, d_tasks, P.nthr,        \
+                               P.thr_tiles, cfg.tile, d_counters, c->d_status);                              \
+        else if (P.thr_ops == OPS_EVALCMP)                                                                   \
+            hipLaunchKernelGGL((k_batch_thread<NN, OPS_EVALCMP>), grid, blk, 0, s, c->d, d_tasks, P.nthr,      \
+                               P.thr_tiles, cfg.tile, d_counters, c->d_status);                              \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_batch_thread<NN, OPS_EVALCMP | OPS_MPINT>), grid, blk, 0, s, c->d, d_tasks,  \
+                               P.nthr, P.thr_tiles, cfg.tile, d_counters, c->d_status);                      \
         break;
             GRNS_BT(4)
             GRNS_BT(8)

@@ -1815,10 +1815,18 @@ GRNS_D void mpint_step(const DevCtx *__restrict__ c, const MpintArgs &A, long lo
     mpint_bounds<OP>(c, sx, sy, x0, x1, y0, y1, r0, r1, minus);
     int nz = 0;
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-        r[i] = mpint_digit<OP>(a[i], b[i], sx, sy, c->m[i], c->dinv[i]);
-        if (OP != 2 && minus) r[i] = r[i] == 0 ? 0 : (c->m[i] - r[i]);
-        nz |= r[i];
+    for (int i0 = 0; i0 < N; i0 += 8) {
+        // 8 moduli of constants at a time (scalar loads here, not hoisted out of the callers' loops)
+        const DevCtx *cc = reload_point(c);
+        cint *m8 = const_view(cc->m + i0);
+        cdouble *v8 = const_view(cc->dinv + i0);
+#pragma unroll
+        for (int k = 0; k < 8 && i0 + k < N; k++) {
+            const int i = i0 + k;
+            r[i] = mpint_digit<OP>(a[i], b[i], sx, sy, m8[k], v8[k]);
+            if (OP != 2 && minus) r[i] = r[i] == 0 ? 0 : (m8[k] - r[i]);
+            nz |= r[i];
+        }
     }
     int sr = (OP == 2) ? ((nz == 0) ? 0 : (sx ^ sy)) : minus;
 #pragma unroll
