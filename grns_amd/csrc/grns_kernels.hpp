@@ -1249,36 +1249,63 @@ __global__ __launch_bounds__(BLOCK) void k_max_mrc_partial(const DevCtx *__restr
 }
 
 // Whether candidate a of stage 1 comes before candidate b: the larger digit vector, then the
-// lower index.
+// lower index.  Digits are compared 16 at a time from the top, each chunk's loads issued together
+// (near-equal candidates share most digits: a digit-by-digit walk would be a chain of dependent
+// loads).
 template <int N>
 GRNS_D bool mrc_part_before(const long long *__restrict__ part_idx, const int *__restrict__ part_dig, int a, int b) {
     const int *da = part_dig + (long long)a * N;
     const int *db = part_dig + (long long)b * N;
+    constexpr int C = N < 16 ? N : 16;
 #pragma unroll 1
-    for (int k = N - 1; k >= 0; k--)
-        if (da[k] != db[k]) return da[k] > db[k];
+    for (int k0 = N - C; k0 >= 0; k0 -= C) {
+        int u[C], v[C];
+#pragma unroll
+        for (int i = 0; i < C; i++) {
+            u[i] = da[k0 + i];
+            v[i] = db[k0 + i];
+        }
+        int r = 0;
+#pragma unroll
+        for (int i = C - 1; i >= 0; i--)
+            if (r == 0 && u[i] != v[i]) r = u[i] > v[i] ? 1 : -1;
+        if (r != 0) return r > 0;
+    }
     return part_idx[a] < part_idx[b];
 }
 
 // Stage 2 (one block): the best of the stage-1 candidates; the winner's residues are copied out.
+// The top digit of every candidate is first loaded into LDS (independent loads); comparisons read
+// it there and go to the full digit rows only when the top digits are equal.
+constexpr int MRC_FINAL_MAX = FBLOCK * MAX_FINAL_PER_THREAD;  // candidates: the arg-max stage-1 grid's bound
+template <int N>
+GRNS_D bool mrc_cand_before(const long long *__restrict__ part_idx, const int *__restrict__ part_dig,
+                            const int *top, int a, int b) {
+    if (top[a] != top[b]) return top[a] > top[b];
+    return mrc_part_before<N>(part_idx, part_dig, a, b);
+}
 template <int N>
 __global__ __launch_bounds__(BLOCK) void k_max_mrc_final(const int *__restrict__ x, long long smod, long long snum,
                                                          const long long *__restrict__ part_idx,
                                                          const int *__restrict__ part_dig, int nparts,
                                                          int *__restrict__ out, long long *__restrict__ out_idx) {
+    __shared__ int top[MRC_FINAL_MAX];
     __shared__ int sk[BLOCK];
     const int t = threadIdx.x;
+    for (int k = t; k < nparts; k += BLOCK)
+        top[k] = part_idx[k] >= 0 ? part_dig[(long long)k * N + N - 1] : 0;
+    __syncthreads();
     int bk = -1;
     for (int k = t; k < nparts; k += BLOCK) {
         if (part_idx[k] < 0) continue;
-        if (bk < 0 || mrc_part_before<N>(part_idx, part_dig, k, bk)) bk = k;
+        if (bk < 0 || mrc_cand_before<N>(part_idx, part_dig, top, k, bk)) bk = k;
     }
     sk[t] = bk;
     __syncthreads();
     for (int h = BLOCK / 2; h >= 1; h >>= 1) {
         if (t < h) {
             const int a = sk[t], b = sk[t + h];
-            if (b >= 0 && (a < 0 || mrc_part_before<N>(part_idx, part_dig, b, a))) sk[t] = b;
+            if (b >= 0 && (a < 0 || mrc_cand_before<N>(part_idx, part_dig, top, b, a))) sk[t] = b;
         }
         __syncthreads();
     }

@@ -746,6 +746,7 @@ grns_status grns_max_mrc(grns_ctx *c, const int *x, long long smod, long long sn
     // the same workspace as grns_max: nb indices, then nb digit vectors (fewer bytes than nb
     // candidates with their residue copies)
     if (!workspace || ws_bytes < max_ws_bytes(c, nb)) return GRNS_ERR_ARG;
+    if (nb > MRC_FINAL_MAX) return GRNS_ERR_ARG;  // never: max_shape caps the grid at this bound
     DeviceGuard guard(c->device);
     long long *pidx = reinterpret_cast<long long *>(workspace);
     int *pdig = reinterpret_cast<int *>(pidx + nb);
