@@ -1022,7 +1022,9 @@ constexpr int MAX_FINAL_PER_THREAD = 8;  // stage 2 takes at most FBLOCK * 8 can
 // against it (keys when they separate, else exactly: in-lane certificates on the residue copies
 // stage 1 made, then wave groups).  Candidates below or equal to the pivot leave (equal ones lower
 // the tie index); those above stay live for the next round.  When none is above, the pivot's value
-// is the maximum and the tie index is the lowest index holding it.
+// is the maximum and the tie index is the lowest index holding it; when exactly one is above, that
+// candidate is the maximum (every other candidate is at most the pivot, and the pivots of earlier
+// rounds are lower still) and holds its value alone.
 template <int N>
 __global__ __launch_bounds__(FBLOCK) void k_max_final(const DevCtx *__restrict__ c, const int *__restrict__ x,
                                                       long long smod, long long snum,
@@ -1032,6 +1034,8 @@ __global__ __launch_bounds__(FBLOCK) void k_max_final(const DevCtx *__restrict__
     __shared__ MaxCand sh[FBLOCK];
     __shared__ int shk[FBLOCK];
     __shared__ unsigned long long s_tie;
+    __shared__ int s_nabove;
+    __shared__ long long s_above_idx;
     const int T = threadIdx.x;
     const int per = (nparts + FBLOCK - 1) / FBLOCK;
     unsigned live = 0;
@@ -1051,7 +1055,10 @@ __global__ __launch_bounds__(FBLOCK) void k_max_final(const DevCtx *__restrict__
             }
         sh[T] = b;
         shk[T] = bk;
-        if (T == 0) s_tie = ~0ull;
+        if (T == 0) {
+            s_tie = ~0ull;
+            s_nabove = 0;
+        }
         __syncthreads();
         for (int h = FBLOCK / 2; h >= 1; h >>= 1) {
             if (T < h && pivot_before(sh[T + h], sh[T])) { sh[T] = sh[T + h]; shk[T] = shk[T + h]; }
@@ -1062,6 +1069,8 @@ __global__ __launch_bounds__(FBLOCK) void k_max_final(const DevCtx *__restrict__
         if (P.idx < 0) break;  // uniform: every candidate has left
         const int *pres = part_res + (long long)pk * N;
         bool above = false;
+        int nabove = 0;
+        long long above_idx = -1;
         unsigned long long tie = (unsigned long long)P.idx;
         for (int t = 0; t < per; t++) {  // uniform trip count: the wave groups may be needed
             const bool mine = live >> t & 1;
@@ -1076,6 +1085,8 @@ __global__ __launch_bounds__(FBLOCK) void k_max_final(const DevCtx *__restrict__
             if (mine) {
                 if (r > 0) {
                     above = true;
+                    nabove++;
+                    above_idx = e.idx;
                 } else {
                     live &= ~(1u << t);
                     if (r == 0 && e.idx != P.idx && (unsigned long long)e.idx < tie) tie = (unsigned long long)e.idx;
@@ -1083,12 +1094,20 @@ __global__ __launch_bounds__(FBLOCK) void k_max_final(const DevCtx *__restrict__
             }
         }
         atomicMin(&s_tie, tie);
+        if (nabove) {
+            atomicAdd(&s_nabove, nabove);
+            s_above_idx = above_idx;  // read only when a single candidate is above (one writer)
+        }
         const bool any_above = __syncthreads_or(above);
         if (!any_above) {
             win = (long long)s_tie;
             break;
         }
-        __syncthreads();  // s_tie and sh are rewritten by the next round
+        if (s_nabove == 1) {
+            win = s_above_idx;
+            break;
+        }
+        __syncthreads();  // s_tie, s_nabove and sh are rewritten by the next round
     }
     if (T < N && win >= 0) out[T] = x[win * snum + (long long)T * smod];
     if (T == 0) *out_idx = win;
