@@ -90,8 +90,9 @@ COPY_BYTES = 1 << 31
 
 def pmc_workload():
     """The program the counter runs profile: a plain 2 GiB device copy (calibration of FETCH_SIZE
-    and WRITE_SIZE), then two launches each of arg-max, evaluation and comparison on the headline
-    workload (n = 32, N = 2^24), and of division and scaling on 2^18 pairs of the n = 64 workload."""
+    and WRITE_SIZE), then two launches each of arg-max (both pipelines), evaluation and comparison on
+    the headline workload (n = 32, N = 2^24), and of division and scaling on 2^18 pairs of the n = 64
+    workload."""
     import torch
 
     import grns_amd as G
@@ -111,6 +112,8 @@ def pmc_workload():
     oi = torch.empty(1, dtype=torch.int64, device="cuda")
     for _ in range(2):
         ctx.max(x, workspace=ws, out=out, out_idx=oi)
+    for _ in range(2):
+        ctx.max(x, workspace=ws, out=out, out_idx=oi, mrc=True)
     torch.cuda.synchronize()
     del x
     torch.cuda.empty_cache()
