@@ -20,10 +20,27 @@
  *   cmp, div, scale2pow, max    - exact integer answers (sign, q/r, floor(x/2^D), argmax)
  *   mpint add/sub/mul           - exact signed integers (KAT -55, 13 from
  *                                 tests/test_verify_mpint.cu:92-93) plus enclosure of evals
- * The interval-bound bit patterns themselves are pinned by no reference test (none stored);
- * they rest on the line-by-line restatement plus the enclosure checks above.
+ * The bit patterns of every floating-point result (eval, eval_fast, the pairwise sums, er_*,
+ * the mpint eval fields) are pinned by tests/test_oracle_bitpins.py against a second model built
+ * from pred/succ (nextafter), round-to-nearest and exact integers, with closed forms for one and
+ * two nonzero leaves, all-zero sums and exact extended-range operands; tests/test_oracle_mutants.py
+ * builds this file with planted mistakes (ORC_MUTANT below) and shows that those pins fail.
  */
 #include "grns_oracle.h"
+
+/* Planted mistakes, for tests/test_oracle_mutants.py only: a build with -DORC_MUTANT=k changes one
+ * step of the restatement so that the tests can show their pins fail on it.  The default build
+ * (ORC_MUTANT 0, oracle/Makefile) has none of them.
+ *   1 tree without the top-level zero pad      2 lower/upper reciprocals swapped in the evaluation
+ *   3 refinement by reciprocal products        4 refinement factor without the +1
+ *   5 integer part dropped with a directed subtraction
+ *   6 fused fmac (std::fma) in BoundNear1      7 sequential instead of pairwise summation
+ *   8 er_add without directed rounding         9 mpint_mul with the two UNIT bounds swapped
+ *  10 no zero test before the sums            11 the ambiguity branch sets upp to INV_UNIT.low
+ *  12 refinement shifts only the upper bound  13 refinement lower bound from the unrefined sums */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
 
 #include <math.h>
 #include <stdlib.h>
@@ -47,7 +64,11 @@ static inline unsigned int orc_to_uint(double x) { return (unsigned int)(long lo
 
 /* ------------------------------------------------------------------ BoundNear1
  * src/dinterval.cuh:25-31 (fmac = x*y+z, EMPLOY_STD_FMA false), :37-147 */
+#if ORC_MUTANT == 6
+static inline double orc_fmac(double x, double y, double z) { return fma(x, y, z); }
+#else
 static inline double orc_fmac(double x, double y, double z) { return x * y + z; }
+#endif
 double orc_dadd_rd(double a, double b) { double c = a + b; return c - orc_fmac(ORC_PHI1, fabs(c), ORC_ETA); }
 double orc_dadd_ru(double a, double b) { double c = a + b; return c + orc_fmac(ORC_PHI1, fabs(c), ORC_ETA); }
 double orc_dsub_rd(double a, double b) { double c = a - b; return c - orc_fmac(ORC_PHI1, fabs(c), ORC_ETA); }
@@ -83,11 +104,19 @@ static double orc_tree_ru(const double *x, int K, int L) {
 }
 static int orc_tree_width(int len) {
     if (len >= 4096) return 8192;
+#if ORC_MUTANT == 1
+    int K = 4;
+    while (K < len) K <<= 1;
+#else
     int K = 8;
     while (K <= len) K <<= 1;
+#endif
     return K;
 }
 double orc_psum_rd(const double *x, int len) {
+#if ORC_MUTANT == 7
+    if (len >= 4) { double a = x[0]; for (int i = 1; i < len; i++) a = orc_dadd_rd(a, x[i]); return a; }
+#endif
     if (len >= 4) return orc_tree_rd(x, orc_tree_width(len), len);
     if (len == 3) return orc_dadd_rd(orc_dadd_rd(x[0], x[1]), x[2]);
     if (len == 2) return orc_dadd_rd(x[0], x[1]);
@@ -95,6 +124,9 @@ double orc_psum_rd(const double *x, int len) {
     return 0;
 }
 double orc_psum_ru(const double *x, int len) {
+#if ORC_MUTANT == 7
+    if (len >= 4) { double a = x[0]; for (int i = 1; i < len; i++) a = orc_dadd_ru(a, x[i]); return a; }
+#endif
     if (len >= 4) return orc_tree_ru(x, orc_tree_width(len), len);
     if (len == 3) return orc_dadd_ru(orc_dadd_ru(x[0], x[1]), x[2]);
     if (len == 2) return orc_dadd_ru(x[0], x[1]);
@@ -123,6 +155,10 @@ void orc_er_adjust(orc_er_t *x) {
     x->frac = u.d * (x->frac != 0);
 }
 /* src/extrange.cuh:153-188; the inputs' exponents are reset when their fraction is zero */
+#if ORC_MUTANT == 8
+#define orc_dadd_rd(a, b) ((a) + (b))
+#define orc_dadd_ru(a, b) ((a) + (b))
+#endif
 void orc_er_add_rd(orc_er_t *r, orc_er_t *x, orc_er_t *y) {
     x->exp *= (x->frac != 0);
     y->exp *= (y->frac != 0);
@@ -145,6 +181,10 @@ void orc_er_add_ru(orc_er_t *r, orc_er_t *x, orc_er_t *y) {
     orc_er_adjust(&res);
     *r = res;
 }
+#if ORC_MUTANT == 8
+#undef orc_dadd_rd
+#undef orc_dadd_ru
+#endif
 /* src/extrange.cuh:213-248 */
 void orc_er_sub_rd(orc_er_t *r, orc_er_t *x, orc_er_t *y) {
     x->exp *= (x->frac != 0);
@@ -416,7 +456,11 @@ static void orc_set_nan(orc_interval_t *out) {
     out->low.frac = NAN; out->low.exp = 0; out->upp.frac = NAN; out->upp.exp = 0;
 }
 static int orc_refine_k(const orc_ctx *c, double sumu) {
+#if ORC_MUTANT == 4
+    double a = -ceil(log2(sumu));
+#else
     double a = -(ceil(log2(sumu)) + 1);
+#endif
     double v = a > c->ref_factor ? a : (double)c->ref_factor;
     if (!(v >= 0 && v <= c->log2M)) return -1;
     return (int)v;
@@ -426,21 +470,37 @@ static void orc_eval_refine(const orc_ctx *c, orc_interval_t *out, int *s, doubl
     int n = c->n;
     double fracl[ORC_MAX_MODULI], fracu[ORC_MAX_MODULI];
     int K = 0, it = 0;
+#if ORC_MUTANT == 13
+    int s0[ORC_MAX_MODULI];
+    memcpy(s0, s, sizeof(int) * (size_t)n);
+#endif
     while (sumu < c->accuracy) {
         int k = orc_refine_k(c, sumu);
         if (k < 0 || ++it > ORC_REFINE_CAP) { orc_set_nan(out); return; }
         orc_rns_mul(c, s, s, c->pow2 + (size_t)k * n);
+#if ORC_MUTANT == 3
+        for (int i = 0; i < n; i++) fracu[i] = orc_dmul_ru(s[i], c->recip_ru[i]);
+#else
         for (int i = 0; i < n; i++) fracu[i] = orc_ddiv_ru((double)s[i], (double)c->moduli[i]);
+#endif
         sumu = orc_psum_ru(fracu, n);
         sumu -= orc_to_uint(sumu);
         K += k;
     }
+#if ORC_MUTANT == 3
+    for (int i = 0; i < n; i++) fracl[i] = orc_dmul_rd(s[i], c->recip_rd[i]);
+#elif ORC_MUTANT == 13
+    for (int i = 0; i < n; i++) fracl[i] = orc_dmul_rd(s0[i], c->recip_rd[i]);
+#else
     for (int i = 0; i < n; i++) fracl[i] = orc_ddiv_rd((double)s[i], (double)c->moduli[i]);
+#endif
     double suml = orc_psum_rd(fracl, n);
     suml -= orc_to_uint(suml);
     orc_er_set_d(&out->low, suml);
     orc_er_set_d(&out->upp, sumu);
+#if ORC_MUTANT != 12
     out->low.exp -= K;
+#endif
     out->upp.exp -= K;
 }
 void orc_eval(const orc_ctx *c, orc_interval_t *out, const int *x) {
@@ -448,28 +508,44 @@ void orc_eval(const orc_ctx *c, orc_interval_t *out, const int *x) {
     int s[ORC_MAX_MODULI], mrd[ORC_MAX_MODULI];
     double fracl[ORC_MAX_MODULI], fracu[ORC_MAX_MODULI];
     int mr = -1;
+#if ORC_MUTANT != 10
     if (orc_check_zero(c, x)) {
         out->low.frac = 0; out->low.exp = 0; out->upp.frac = 0; out->upp.exp = 0;
         return;
     }
+#endif
     orc_rns_mul(c, s, x, c->part_inv);
     for (int i = 0; i < n; i++) {
+#if ORC_MUTANT == 2
+        fracl[i] = orc_dmul_rd(s[i], c->recip_ru[i]);
+        fracu[i] = orc_dmul_ru(s[i], c->recip_rd[i]);
+#else
         fracl[i] = orc_dmul_rd(s[i], c->recip_rd[i]);
         fracu[i] = orc_dmul_ru(s[i], c->recip_ru[i]);
+#endif
     }
     double suml = orc_psum_rd(fracl, n);
     double sumu = orc_psum_ru(fracu, n);
     unsigned int whl = orc_to_uint(suml);
     unsigned int whu = orc_to_uint(sumu);
+#if ORC_MUTANT == 5
+    suml = orc_dsub_rd(suml, whl);
+    sumu = orc_dsub_ru(sumu, whu);
+#else
     suml = suml - whl;
     sumu = sumu - whu;
+#endif
     orc_er_set_d(&out->low, suml);
     orc_er_set_d(&out->upp, sumu);
     if (whl != whu) {
         orc_mrc(c, mrd, x);
         mr = mrd[n - 1];
     }
+#if ORC_MUTANT == 11
+    if (mr > 0) { out->upp = c->inv_unit.low; return; }
+#else
     if (mr > 0) { out->upp = c->inv_unit.upp; return; }
+#endif
     if (mr == 0) out->low = c->unit.low;
     if (sumu >= c->accuracy) return;
     orc_eval_refine(c, out, s, sumu);
@@ -709,8 +785,13 @@ int orc_mpint_mul(const orc_ctx *c, int *rd, int *rs, orc_er_t *re,
     int n = c->n;
     orc_er_t x0 = xe[0], x1 = xe[1], y0 = ye[0], y1 = ye[1];
     orc_er_t uu = c->unit.upp, ul = c->unit.low;
+#if ORC_MUTANT == 9
+    orc_er_md_rd(&re[0], &x0, &y0, &ul);
+    orc_er_md_ru(&re[1], &x1, &y1, &uu);
+#else
     orc_er_md_rd(&re[0], &x0, &y0, &uu);
     orc_er_md_ru(&re[1], &x1, &y1, &ul);
+#endif
     for (int i = 0; i < n; i++) rd[i] = orc_mod_mul(xd[i], yd[i], c->moduli[i], 1.0 / c->moduli[i]);
     *rs = orc_check_zero(c, rd) ? 0 : xs ^ ys;
     return orc_check_overflow(c, &re[0], &re[1]);

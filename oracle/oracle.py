@@ -60,15 +60,25 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def build_mutant(k: int, out_dir: str) -> str:
+    """Compile the restatement with planted mistake k (-DORC_MUTANT=k, listed in grns_oracle.c) into
+    out_dir; used only by the tests that show the pins fail on a wrong restatement."""
+    out = os.path.join(out_dir, "liboracle_mutant%d.so" % k)
+    subprocess.run(["gcc", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-std=gnu11", "-shared",
+                    "-DORC_MUTANT=%d" % k, "-o", out, os.path.join(_HERE, "grns_oracle.c"), "-lm"], check=True)
+    return out
+
+
+def lib(path: str | None = None):
+    """The restatement's shared library (the default build, or another build of it at `path`)."""
+    key = path or _LIB_PATH
+    if key not in _libs:
+        if path is None and not os.path.exists(_LIB_PATH):
             build()
-        L = C.CDLL(_LIB_PATH)
+        L = C.CDLL(key)
         P = C.c_void_p
         L.orc_ctx_create.restype = C.POINTER(OrcCtx)
         L.orc_ctx_create.argtypes = [P, C.c_int]
@@ -130,8 +140,8 @@ def lib():
         L.orc_max_eval_array.argtypes = [CP, P, C.c_size_t, C.c_size_t, C.c_int]
         L.orc_mpint_array.argtypes = [CP, C.c_int, C.c_size_t, C.c_size_t, P, P, P, P, P, P, P, P, P, P, C.c_int]
         L.orc_mpint_div_array.argtypes = [CP, C.c_size_t, C.c_size_t, P, P, P, P, P, P, P, C.c_int]
-        _lib = L
-    return _lib
+        _libs[key] = L
+    return _libs[key]
 
 
 def _ptr(a: np.ndarray):
@@ -145,18 +155,19 @@ def _i32(a) -> np.ndarray:
 class Oracle:
     """One moduli set of the CPU restatement."""
 
-    def __init__(self, moduli):
+    def __init__(self, moduli, lib_path: str | None = None):
         self.moduli = [int(v) for v in moduli]
         self.n = len(self.moduli)
+        self._L = lib(lib_path)
         arr = _i32(self.moduli)
-        self._p = lib().orc_ctx_create(_ptr(arr), self.n)
+        self._p = self._L.orc_ctx_create(_ptr(arr), self.n)
         if not self._p:
             raise ValueError("bad moduli set")
         self.c = self._p.contents
 
     def close(self):
         if self._p:
-            lib().orc_ctx_destroy(self._p)
+            self._L.orc_ctx_destroy(self._p)
             self._p = None
 
     def __del__(self):
@@ -198,27 +209,27 @@ class Oracle:
     def mrc(self, x):
         a = _i32(x)
         out = np.zeros(self.n, np.int32)
-        lib().orc_mrc(self._p, _ptr(out), _ptr(a))
+        self._L.orc_mrc(self._p, _ptr(out), _ptr(a))
         return out
 
     def eval1(self, x, fast=False):
         a = _i32(x)
         out = np.zeros(1, INTERVAL_DTYPE)
-        (lib().orc_eval_fast if fast else lib().orc_eval)(self._p, _ptr(out), _ptr(a))
+        (self._L.orc_eval_fast if fast else self._L.orc_eval)(self._p, _ptr(out), _ptr(a))
         return out[0]
 
     def cmp1(self, x, y):
-        return lib().orc_cmp(self._p, _ptr(_i32(x)), _ptr(_i32(y)))
+        return self._L.orc_cmp(self._p, _ptr(_i32(x)), _ptr(_i32(y)))
 
     def div1(self, x, d, fast=True):
         q = np.zeros(self.n, np.int32)
         r = np.zeros(self.n, np.int32)
-        it = (lib().orc_div_fast if fast else lib().orc_div)(self._p, _ptr(q), _ptr(r), _ptr(_i32(x)), _ptr(_i32(d)))
+        it = (self._L.orc_div_fast if fast else self._L.orc_div)(self._p, _ptr(q), _ptr(r), _ptr(_i32(x)), _ptr(_i32(d)))
         return q, r, it
 
     def scale1(self, x, D):
         out = np.zeros(self.n, np.int32)
-        lib().orc_scale2pow(self._p, _ptr(out), _ptr(_i32(x)), int(D))
+        self._L.orc_scale2pow(self._p, _ptr(out), _ptr(_i32(x)), int(D))
         return out
 
     # --- arrays (modulus-major (n, count)) -------------------------------------------
@@ -226,14 +237,14 @@ class Oracle:
         x = _i32(x)
         cnt = x.shape[1]
         out = np.zeros(cnt, INTERVAL_DTYPE)
-        lib().orc_eval_array(self._p, _ptr(x), cnt, cnt, _ptr(out), int(fast), nthreads)
+        self._L.orc_eval_array(self._p, _ptr(x), cnt, cnt, _ptr(out), int(fast), nthreads)
         return out
 
     def cmp_array(self, x, y, nthreads=0):
         x, y = _i32(x), _i32(y)
         cnt = x.shape[1]
         out = np.zeros(cnt, np.int32)
-        lib().orc_cmp_array(self._p, _ptr(x), _ptr(y), cnt, cnt, _ptr(out), nthreads)
+        self._L.orc_cmp_array(self._p, _ptr(x), _ptr(y), cnt, cnt, _ptr(out), nthreads)
         return out
 
     def div_array(self, x, d, fast=True, nthreads=0):
@@ -241,7 +252,7 @@ class Oracle:
         cnt = x.shape[1]
         q = np.zeros_like(x)
         r = np.zeros_like(x)
-        lib().orc_div_array(self._p, _ptr(x), _ptr(d), cnt, cnt, _ptr(q), _ptr(r), int(fast), nthreads)
+        self._L.orc_div_array(self._p, _ptr(x), _ptr(d), cnt, cnt, _ptr(q), _ptr(r), int(fast), nthreads)
         return q, r
 
     def scale2pow_array(self, x, D, nthreads=0):
@@ -249,21 +260,21 @@ class Oracle:
         D = np.ascontiguousarray(D, dtype=np.uint32)
         cnt = x.shape[1]
         y = np.zeros_like(x)
-        lib().orc_scale2pow_array(self._p, _ptr(x), _ptr(D), cnt, cnt, _ptr(y), nthreads)
+        self._L.orc_scale2pow_array(self._p, _ptr(x), _ptr(D), cnt, cnt, _ptr(y), nthreads)
         return y
 
     def max_array(self, x, nthreads=0):
         x = _i32(x)
         cnt = x.shape[1]
         res = np.zeros(self.n, np.int32)
-        idx = lib().orc_max_array(self._p, _ptr(x), cnt, cnt, _ptr(res), nthreads)
+        idx = self._L.orc_max_array(self._p, _ptr(x), cnt, cnt, _ptr(res), nthreads)
         return idx, res
 
     def max_eval_array(self, x, nthreads=0):
         """Arg-max by the reference's evaluate-then-reduce pipeline (CPU path)."""
         x = _i32(x)
         cnt = x.shape[1]
-        return lib().orc_max_eval_array(self._p, _ptr(x), cnt, cnt, nthreads)
+        return self._L.orc_max_eval_array(self._p, _ptr(x), cnt, cnt, nthreads)
 
     def mpint_div_array(self, xd, xs, yd, ys, nthreads=0):
         """mpint_div (Euclidean) of the magnitudes xd / yd with signs: (digits, sign, eval)."""
@@ -273,7 +284,7 @@ class Oracle:
         rd = np.zeros_like(xd)
         rs = np.zeros(cnt, np.int32)
         re = np.zeros((cnt, 2), ER_DTYPE)
-        lib().orc_mpint_div_array(self._p, cnt, cnt, _ptr(xd), _ptr(xs), _ptr(yd), _ptr(ys), _ptr(rd), _ptr(rs),
+        self._L.orc_mpint_div_array(self._p, cnt, cnt, _ptr(xd), _ptr(xs), _ptr(yd), _ptr(ys), _ptr(rd), _ptr(rs),
                                   _ptr(re), nthreads)
         return rd, rs, re
 
@@ -289,6 +300,6 @@ class Oracle:
         rs = np.zeros(cnt, np.int32)
         re = np.zeros((cnt, 2), ER_DTYPE)
         ovf = np.zeros(cnt, np.uint8)
-        lib().orc_mpint_array(self._p, opc, cnt, cnt, _ptr(xd), _ptr(xs), _ptr(xe), _ptr(yd), _ptr(ys), _ptr(ye),
+        self._L.orc_mpint_array(self._p, opc, cnt, cnt, _ptr(xd), _ptr(xs), _ptr(xe), _ptr(yd), _ptr(ys), _ptr(ye),
                               _ptr(rd), _ptr(rs), _ptr(re), _ptr(ovf), nthreads)
         return rd, rs, re, ovf
