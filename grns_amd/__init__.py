@@ -21,6 +21,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgrns_hip.so")
 
 GRNS_OK = 0
+GRNS_ERR_ARG = 1
+GRNS_ERR_UNSUPPORTED = 4
 STATUS_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_OOM", 4: "ERR_UNSUPPORTED", 5: "ERR_ADMISSION",
                 6: "ERR_BUSY"}
 
@@ -67,7 +69,7 @@ _lib = None
 EXPORTS = [
     "grns_version", "grns_ctx_create", "grns_ctx_destroy", "grns_ctx_n", "grns_ctx_log2m", "grns_ctx_constants",
     "grns_ctx_tables", "grns_ctx_status", "grns_eval", "grns_cmp", "grns_div", "grns_scale2pow",
-    "grns_max_workspace_size", "grns_max", "grns_max_mrc", "grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul", "grns_mpint_div",
+    "grns_max_workspace_size", "grns_max", "grns_max_noalloc", "grns_max_mrc", "grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul", "grns_mpint_div",
     "grns_batch_create", "grns_batch_set_config", "grns_batch_get_config", "grns_batch_tune", "grns_batch_launch",
     "grns_batch_destroy", "grns_serve_create", "grns_serve_submit", "grns_serve_poll", "grns_serve_wait",
     "grns_serve_drain", "grns_serve_stats_get", "grns_serve_destroy", "grns_ctx_words", "grns_from_binary",
@@ -112,6 +114,7 @@ def lib():
     L.grns_to_binary.argtypes = [P, P, LL, LL, SZ, P, LL, LL, P]
     L.grns_max_workspace_size.argtypes = [P, SZ]
     L.grns_max.argtypes = [P, P, LL, LL, SZ, P, P, P, SZ, P]
+    L.grns_max_noalloc.argtypes = [P, P, LL, LL, SZ, P, P, P]
     L.grns_max_mrc.argtypes = [P, P, LL, LL, SZ, P, P, P, SZ, P]
     for nm in ("grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul", "grns_mpint_div"):
         getattr(L, nm).argtypes = [P, C.POINTER(MpintArray), C.POINTER(MpintArray), C.POINTER(MpintArray), SZ, P, P]

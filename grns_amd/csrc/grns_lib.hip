@@ -315,6 +315,36 @@ void stage_end(grns_ctx *c, int op, int k, void *stream) {
     if (hipEventRecord((hipEvent_t)t.ev[2 * k + 1], S(stream)) == hipSuccess) t.pending[k] = 1;
 }
 
+// Scratch of grns_max_noalloc: the stage-1 candidates and their residue copies in a file-scope
+// device array instead of a caller's workspace.  Every GPU the library runs on holds its own copy
+// (GRNS_MAX_NOALLOC_SCRATCH_BYTES, until the process ends) at its own address, which
+// hipGetSymbolAddress resolves once per device when the first context on it is made; host code
+// uses that address, never the array's name.  Calls that overlap on one GPU share the array and race.
+static_assert(GRNS_MAX_NOALLOC_SCRATCH_BYTES >= (size_t)FBLOCK * MAX_FINAL_PER_THREAD * (sizeof(MaxCand) + 64 * sizeof(int)),
+              "scratch holds the largest stage-1 grid at n = 64");
+static __device__ __align__(256) unsigned char scratch[GRNS_MAX_NOALLOC_SCRATCH_BYTES];
+
+constexpr int SCRATCH_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_at[SCRATCH_DEVICES];
+
+// Resolves this device's address of `scratch` (the device must be current).
+static grns_status scratch_setup(int device) {
+    if (device < 0 || device >= SCRATCH_DEVICES) return GRNS_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_at[device]) return GRNS_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GRNS_ERR_HIP;
+    g_scratch_at[device] = static_cast<unsigned char *>(p);
+    return GRNS_OK;
+}
+
+static unsigned char *scratch_ptr(int device) {
+    if (device < 0 || device >= SCRATCH_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_at[device];
+}
+
 }  // namespace
 
 extern "C" {
@@ -360,6 +390,10 @@ grns_status grns_ctx_create(const int *moduli, int n, int device, grns_ctx **out
         return GRNS_ERR_HIP;
     }
     c->max_blocks = max_grid_for(c);
+    if (scratch_setup(c->device) != GRNS_OK) {
+        grns_ctx_destroy(c);
+        return GRNS_ERR_HIP;
+    }
     *out = c;
     return GRNS_OK;
 }
@@ -698,28 +732,22 @@ size_t grns_max_workspace_size(const grns_ctx *c, size_t count) {
     return max_ws_bytes(c, nb);
 }
 
-grns_status grns_max(grns_ctx *c, const int *x, long long smod, long long snum, size_t count, int *out_residues,
-                     long long *out_idx, void *workspace, size_t ws_bytes, void *stream) {
-    if (c && c->device < 0) return GRNS_ERR_ARG;
-    if (!c || !x || !out_residues || !out_idx || count == 0) return GRNS_ERR_ARG;
-    long long nb, ch;
-    max_shape(c, count, nb, ch);
-    if (!workspace || ws_bytes < max_ws_bytes(c, nb)) return GRNS_ERR_ARG;
-    DeviceGuard guard(c->device);
-    MaxCand *part = reinterpret_cast<MaxCand *>(workspace);
-    int *part_res = reinterpret_cast<int *>(part + nb);
+// Both arg-max stages over the candidates part[nb] and their residue copies part_res[nb * n].
+static grns_status max_run(grns_ctx *c, const int *x, long long smod, long long snum, size_t count, int *out_residues,
+                           long long *out_idx, MaxCand *part, int *part_res, long long nb, long long ch, void *stream,
+                           bool timed) {
     const bool mm = snum == 1 && count <= ((size_t)1 << 29);
     dim3 grid((unsigned)nb), blk(BLOCK), fblk(FBLOCK);
     long long cnt = (long long)count;
-    int tk;
+    int tk = -1;
 #define GRNS_MAX_CASE(NN)                                                                                    \
     case NN:                                                                                                 \
-        tk = stage_begin(c, 2, stream);                                                                      \
+        if (timed) tk = stage_begin(c, 2, stream);                                                           \
         if (mm) hipLaunchKernelGGL((k_max_partial<NN, LD_MM>), grid, blk, 0, S(stream), c->d, x, smod, snum, \
                                    cnt, ch, part, part_res);                                                 \
         else hipLaunchKernelGGL((k_max_partial<NN, LD_ANY>), grid, blk, 0, S(stream), c->d, x, smod, snum,    \
                                 cnt, ch, part, part_res);                                                    \
-        stage_end(c, 2, tk, stream);                                                                         \
+        if (timed) stage_end(c, 2, tk, stream);                                                              \
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                            \
         hipLaunchKernelGGL((k_max_final<NN>), dim3(1), fblk, 0, S(stream), c->d, x, smod, snum, part,        \
                            part_res, (int)nb, out_residues, out_idx);                                        \
@@ -735,6 +763,35 @@ grns_status grns_max(grns_ctx *c, const int *x, long long smod, long long snum, 
     }
 #undef GRNS_MAX_CASE
     return launch_check();
+}
+
+grns_status grns_max(grns_ctx *c, const int *x, long long smod, long long snum, size_t count, int *out_residues,
+                     long long *out_idx, void *workspace, size_t ws_bytes, void *stream) {
+    if (c && c->device < 0) return GRNS_ERR_ARG;
+    if (!c || !x || !out_residues || !out_idx || count == 0) return GRNS_ERR_ARG;
+    long long nb, ch;
+    max_shape(c, count, nb, ch);
+    if (!workspace || ws_bytes < max_ws_bytes(c, nb)) return GRNS_ERR_ARG;
+    DeviceGuard guard(c->device);
+    MaxCand *part = reinterpret_cast<MaxCand *>(workspace);
+    int *part_res = reinterpret_cast<int *>(part + nb);
+    return max_run(c, x, smod, snum, count, out_residues, out_idx, part, part_res, nb, ch, stream, true);
+}
+
+grns_status grns_max_noalloc(grns_ctx *c, const int *x, long long smod, long long snum, size_t count,
+                             int *out_residues, long long *out_idx, void *stream) {
+    if (c && c->device < 0) return GRNS_ERR_ARG;
+    if (!c || !x || !out_residues || !out_idx || count == 0) return GRNS_ERR_ARG;
+    if (c->n != 4 && c->n != 8 && c->n != 16 && c->n != 32 && c->n != 64) return GRNS_ERR_UNSUPPORTED;
+    long long nb, ch;
+    max_shape(c, count, nb, ch);
+    if (max_ws_bytes(c, nb) > GRNS_MAX_NOALLOC_SCRATCH_BYTES) return GRNS_ERR_ARG;
+    unsigned char *base = scratch_ptr(c->device);
+    if (!base) return GRNS_ERR_HIP;
+    DeviceGuard guard(c->device);
+    MaxCand *part = reinterpret_cast<MaxCand *>(base);
+    int *part_res = reinterpret_cast<int *>(part + nb);
+    return max_run(c, x, smod, snum, count, out_residues, out_idx, part, part_res, nb, ch, stream, false);
 }
 
 grns_status grns_max_mrc(grns_ctx *c, const int *x, long long smod, long long snum, size_t count, int *out_residues,

@@ -111,6 +111,19 @@ size_t grns_max_workspace_size(const grns_ctx *ctx, size_t count);
 grns_status grns_max(grns_ctx *ctx, const int *x, long long mod_stride, long long num_stride, size_t count,
                      int *out_residues, long long *out_idx, void *workspace, size_t workspace_bytes, void *stream);
 
+/* grns_max without the workspace argument, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart
+ * in the reference.  The stage-1 candidates live in a file-scope device array of
+ * GRNS_MAX_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends; the call allocates
+ * nothing.  Same arguments (minus the workspace) and results as grns_max; GRNS_ERR_ARG when the
+ * candidates of `count` numbers would not fit the array (never for n <= 64), GRNS_ERR_UNSUPPORTED
+ * for moduli-set sizes other than 4, 8, 16, 32, 64.  Every call on one GPU uses the same array:
+ * calls that overlap on that GPU (other streams, other threads) race and must be serialised by
+ * the caller. */
+#define GRNS_MAX_NOALLOC_SCRATCH_BYTES ((size_t)2293760)  /* 8192 candidates x (24 + 4 * 64) bytes */
+grns_status grns_max_noalloc(grns_ctx *ctx, const int *x, long long mod_stride, long long num_stride, size_t count,
+                             int *out_residues, long long *out_idx, void *stream);
+
 /* Arg-max by mixed-radix digits: rns_max_mrc (tests/test_perf_rnsmax.cu:145-153, with
  * rns_max_mrc_compute_kernel and rns_max_mrc_tree_kernel :109-138).  Every number is converted to
  * mixed-radix digits (src/mrc.cuh:31-44) and the largest digit vector wins (mrs_cmp,
