@@ -23,6 +23,8 @@ LIB_PATH = os.path.join(_HERE, "libgrns_hip.so")
 GRNS_OK = 0
 GRNS_ERR_ARG = 1
 GRNS_ERR_UNSUPPORTED = 4
+STATUS_REFINE_RANGE = 1  # GRNS_STATUS_REFINE_RANGE: a refinement factor outside RNS_POW2
+STATUS_DIV_LOOP = 2      # GRNS_STATUS_DIV_LOOP: a division loop that would not terminate
 STATUS_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_OOM", 4: "ERR_UNSUPPORTED", 5: "ERR_ADMISSION",
                 6: "ERR_BUSY"}
 
@@ -270,7 +272,9 @@ class Context:
                               _stream_ptr(stream)), "grns_cmp")
         return out
 
-    def div(self, x, d, fast=True, q=None, r=None, stream=None):
+    def div(self, x, d, fast=True, q=None, r=None, stream=None, reference=False):
+        """Quotients and remainders (grns_div); reference=True runs the reference's loop for every
+        pair (GRNS_DIV_REFERENCE: the CPU path's results bit for bit, wrapped ones included)."""
         import torch
         if q is None:
             q = torch.empty_like(x)
@@ -278,7 +282,8 @@ class Context:
             r = torch.empty_like(x)
         sm, sn = self._strides(x)
         assert self._strides(d) == (sm, sn) and self._strides(q) == (sm, sn) and self._strides(r) == (sm, sn)
-        _check(lib().grns_div(self._h, _tptr(x), _tptr(d), sm, sn, x.shape[1], _tptr(q), _tptr(r), int(fast),
+        flags = (1 if fast else 0) | (2 if reference else 0)
+        _check(lib().grns_div(self._h, _tptr(x), _tptr(d), sm, sn, x.shape[1], _tptr(q), _tptr(r), flags,
                               _stream_ptr(stream)), "grns_div")
         return q, r
 
@@ -337,12 +342,16 @@ class Context:
         pipeline (grns_max_mrc), which returns the same result."""
         import torch
         cnt = x.shape[1]
-        if workspace is None:
-            workspace = torch.empty(self.max_workspace_size(cnt), dtype=torch.uint8, device=x.device)
-        if out is None:
-            out = torch.empty(self.n, dtype=torch.int32, device=x.device)
-        if out_idx is None:
-            out_idx = torch.empty(1, dtype=torch.int64, device=x.device)
+        launch = stream if stream is not None else torch.cuda.current_stream()
+        # temporaries belong to the launch stream: the caching allocator hands a freed block out
+        # again only to work ordered after that stream's kernels
+        with torch.cuda.stream(launch):
+            if workspace is None:
+                workspace = torch.empty(self.max_workspace_size(cnt), dtype=torch.uint8, device=x.device)
+            if out is None:
+                out = torch.empty(self.n, dtype=torch.int32, device=x.device)
+            if out_idx is None:
+                out_idx = torch.empty(1, dtype=torch.int64, device=x.device)
         sm, sn = self._strides(x)
         fn, name = (lib().grns_max_mrc, "grns_max_mrc") if mrc else (lib().grns_max, "grns_max")
         _check(fn(self._h, _tptr(x), sm, sn, cnt, _tptr(out), _tptr(out_idx), _tptr(workspace), workspace.numel(),
@@ -409,8 +418,13 @@ class Context:
 
 def make_task(op, count, a=None, b=None, out0=None, out1=None, D=None, mx=None, my=None, mz=None, overflow=None):
     """grns_batch_task from device tensors.  a/b/out0/out1: (n, count)-shaped residue views
-    (or the interval / sign outputs); mx/my/mz: (digits, sign, eval) triples for mpint ops."""
+    (or the interval / sign outputs); mx/my/mz: (digits, sign, eval) triples for mpint ops.
+    The task holds references to its tensors (`_keep`): Batch keeps its tasks for its lifetime and
+    Server keeps a request's tasks until the request completes, so the device memory they point to
+    cannot be freed while a launch may still read or write it."""
     t = BatchTask()
+    t._keep = tuple(v for v in (a, b, out0, out1, D, overflow) if v is not None) + \
+        tuple(v for tri in (mx, my, mz) if tri is not None for v in tri)
     t.op = OPS[op] if isinstance(op, str) else int(op)
     t.count = int(count)
     if a is not None:
@@ -501,35 +515,47 @@ class Server:
         L.grns_serve_destroy.argtypes = [P]
         cfg = ServeConfig(hbm_budget, slots, max_batch, tile, blocks_per_cu)
         h = C.c_void_p()
+        self._inflight = {}  # ticket -> (task array holding the request's tensors, context)
         _check(L.grns_serve_create(device, C.byref(cfg), C.byref(h)), "grns_serve_create")
         self._h = h
 
     @staticmethod
     def task_array(tasks):
-        """The ctypes task table of a request (build it once to resubmit a request cheaply)."""
-        return (BatchTask * len(tasks))(*tasks)
+        """The ctypes task table of a request (build it once to resubmit a request cheaply); it holds
+        the tasks' tensor references."""
+        arr = (BatchTask * len(tasks))(*tasks)
+        arr._keep = tuple(getattr(t, "_keep", ()) for t in tasks)
+        return arr
 
     def submit(self, ctx: Context, tasks):
-        """Queue a request (a list of make_task() tasks or a task_array()); returns its ticket."""
+        """Queue a request (a list of make_task() tasks or a task_array()); returns its ticket.  The
+        request's tensors (and its context) stay referenced until wait() or poll() sees it complete,
+        or drain() or close() returns."""
         arr = tasks if isinstance(tasks, C.Array) else self.task_array(tasks)
         t = C.c_longlong()
         rc = lib().grns_serve_submit(self._h, ctx.handle, arr, len(arr), C.byref(t))
         if rc != GRNS_OK:
             raise GrnsError(rc, "grns_serve_submit")
+        self._inflight[t.value] = (arr, ctx)
         return t.value
 
     def poll(self, ticket):
         st, lat = C.c_int(), C.c_double()
         rc = lib().grns_serve_poll(self._h, ticket, C.byref(st), C.byref(lat))
+        if rc == GRNS_OK and st.value in (2, 3):  # done or failed: nothing of it runs any more
+            self._inflight.pop(ticket, None)
         return st.value, lat.value, rc
 
     def wait(self, ticket):
         lat = C.c_double()
-        _check(lib().grns_serve_wait(self._h, ticket, C.byref(lat)), "grns_serve_wait")
+        rc = lib().grns_serve_wait(self._h, ticket, C.byref(lat))
+        self._inflight.pop(ticket, None)  # wait returns once the request has finished, either way
+        _check(rc, "grns_serve_wait")
         return lat.value
 
     def drain(self):
         _check(lib().grns_serve_drain(self._h), "grns_serve_drain")
+        self._inflight.clear()
 
     def stats(self):
         s = ServeStats()
@@ -538,8 +564,9 @@ class Server:
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
-            lib().grns_serve_destroy(self._h)
+            lib().grns_serve_destroy(self._h)  # drains the requests still queued or running
             self._h = C.c_void_p()
+            self._inflight.clear()
 
     def __del__(self):
         try:

@@ -359,28 +359,38 @@ grns_status enqueue(grns_batch *b, hipStream_t s) {
     return enqueue_plan(b->ctx, b->plan, b->d_tasks, b->d_counters, b->cfg, b->cus, s);
 }
 
+// Every pointer a task's operation reads or writes is set, and mpint arrays are as long as the
+// task (the checks of the per-call entry points).  The serving form (allow_max) also takes the
+// arg-max.  A task of count 0 is valid whatever its pointers.
+bool task_valid(const grns_batch_task &u, bool allow_max) {
+    if (u.op < GRNS_OP_EVAL || u.op > (allow_max ? GRNS_OP_MAX : GRNS_OP_SCALE2POW)) return false;
+    if (u.count == 0) return true;
+    switch (u.op) {
+        case GRNS_OP_MPINT_ADD:
+        case GRNS_OP_MPINT_SUB:
+        case GRNS_OP_MPINT_MUL: {
+            const grns_mpint_array *m[3] = {&u.mx, &u.my, &u.mz};
+            for (const grns_mpint_array *a : m)
+                if (!a->digits || !a->sign || !a->eval || a->ld != u.mx.ld) return false;
+            return u.mx.ld >= (long long)u.count;
+        }
+        case GRNS_OP_CMP: return u.a && u.b && u.out0;
+        case GRNS_OP_DIV:
+        case GRNS_OP_DIV_FAST: return u.a && u.b && u.out0 && u.out1;
+        case GRNS_OP_SCALE2POW: return u.a && u.D && u.out0;
+        case GRNS_OP_MAX: return u.a && u.out0 && u.out1;
+        default: return u.a && u.out0;  // eval, eval_fast
+    }
+}
+
 }  // namespace
 
 extern "C" {
 
 grns_status grns_batch_create(grns_ctx *ctx, const grns_batch_task *tasks, int ntasks, grns_batch **out) {
     if (!ctx || !out || ntasks < 0 || (ntasks && !tasks) || ctx->device < 0 || ctx->n > 64) return GRNS_ERR_ARG;
-    for (int i = 0; i < ntasks; i++) {
-        const grns_batch_task &u = tasks[i];
-        if (u.op < GRNS_OP_EVAL || u.op > GRNS_OP_SCALE2POW) return GRNS_ERR_ARG;  // arg-max: use grns_max
-        if (u.count == 0) continue;
-        bool mp = u.op >= GRNS_OP_MPINT_ADD && u.op <= GRNS_OP_MPINT_MUL;
-        if (mp) {
-            if (!u.mx.digits || !u.my.digits || !u.mz.digits || u.mx.ld != u.my.ld || u.mx.ld != u.mz.ld ||
-                u.mx.ld < (long long)u.count)
-                return GRNS_ERR_ARG;
-        } else {
-            if (!u.a || !u.out0) return GRNS_ERR_ARG;
-            if ((u.op == GRNS_OP_CMP || u.op == GRNS_OP_DIV || u.op == GRNS_OP_DIV_FAST) && !u.b) return GRNS_ERR_ARG;
-            if ((u.op == GRNS_OP_DIV || u.op == GRNS_OP_DIV_FAST) && !u.out1) return GRNS_ERR_ARG;
-            if (u.op == GRNS_OP_SCALE2POW && !u.D) return GRNS_ERR_ARG;
-        }
-    }
+    for (int i = 0; i < ntasks; i++)
+        if (!task_valid(tasks[i], false)) return GRNS_ERR_ARG;  // arg-max: use grns_max
     grns_batch *b = new (std::nothrow) grns_batch();
     if (!b) return GRNS_ERR_OOM;
     b->ctx = ctx;
@@ -796,8 +806,7 @@ grns_status grns_serve_submit(grns_server *S, grns_ctx *ctx, const grns_batch_ta
     r.bytes = 0;
     for (int i = 0; i < ntasks; i++) {
         const grns_batch_task &u = tasks[i];
-        if (u.op < GRNS_OP_EVAL || u.op > GRNS_OP_MAX) return GRNS_ERR_ARG;
-        if (u.op == GRNS_OP_MAX && u.count > 0 && (!u.a || !u.out0 || !u.out1)) return GRNS_ERR_ARG;
+        if (!task_valid(u, true)) return GRNS_ERR_ARG;
         r.bytes += task_bytes(u, ctx->n);
     }
     if (ntasks > 8 * S->cfg.max_batch) return GRNS_ERR_ARG;

@@ -108,6 +108,7 @@ struct interval_t {
 
 constexpr int MAXN = 256;      // largest moduli set a context accepts
 constexpr int THRESH = 30;     // RNS_P2_SCALING_THRESHOLD, src/params.h:22
+constexpr int KEY_SHIFT = 26;  // arg-max keys of values near 0 or M: the key of 2^KEY_SHIFT x
 
 // Per-moduli-set constants resident in HBM (rns_const_init, src/rnsbase.cuh:324-451, plus the
 // tables this build's filtered paths need).
@@ -133,6 +134,8 @@ struct DevCtx {
     double clo[MAXN];
     double khi[MAXN];            // w_i / m_i truncated to 22 significant bits: x_i * khi is exact
     double klo[MAXN];            // w_i / m_i - khi (arg-max keys)
+    double shi[MAXN];            // as khi, klo for (2^KEY_SHIFT w_i mod m_i) / m_i: the key of
+    double slo[MAXN];            //   2^KEY_SHIFT x, for values within 2^-28 M of 0 or M
     int c32[MAXN];               // 2^32 mod m_i
     unsigned long long mi64[MAXN];  // (M / m_i) mod 2^64
     unsigned long long M64;         // M mod 2^64

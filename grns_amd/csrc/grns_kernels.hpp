@@ -342,14 +342,15 @@ GRNS_HD double filter_frac(const DevCtx *c, const int (&xr)[N], bool &regular, b
 // sums) < 2^-31 up to n = 64.  Five VALU operations per residue.
 // The sum depends on the residues only mod m_i (w_i/m_i times a multiple of m_i is an integer), so
 // the key holds for non-canonical int32 residues too, which then stand for their canonical value.
-template <int N>
+// SCALED: the key of 2^KEY_SHIFT x (constants shi, slo), for the arg-max keys of values near 0 or M.
+template <int N, bool SCALED = false>
 GRNS_HD double key_frac(const DevCtx *c, const int (&xr)[N]) {
     double A = 0.0, B = 0.0;
 #pragma unroll
     for (int i0 = 0; i0 < N; i0 += 8) {
         const DevCtx *cc = reload_point(c);  // 8 moduli of constants at a time
-        cdouble *kh = const_view(cc->khi + i0);
-        cdouble *kl = const_view(cc->klo + i0);
+        cdouble *kh = const_view((SCALED ? cc->shi : cc->khi) + i0);
+        cdouble *kl = const_view((SCALED ? cc->slo : cc->klo) + i0);
 #pragma unroll
         for (int k = 0; k < 8 && i0 + k < N; k++) {
             double dx = (double)xr[i0 + k];
@@ -927,14 +928,52 @@ GRNS_D void coop_resolve(const DevCtx *c, const int *__restrict__ x, long long s
     if (mask) r = coop_resolve_lanes<N>(c, x, smod, snum, a, b, r, mask);
 }
 
+// Unknown key (x/M may lie within 3E < 2^-28 of 0 or of 1, where F cannot tell the two ends
+// apart) made known from the residues in memory, on the rare path of a merge: the key of
+// 2^KEY_SHIFT x, H = frac(2^26 x / M) (error at most E, as F's), is below 3/16 + E for x < 3E M and
+// above 13/16 - E for x > (1 - 3E) M, and gives x/M = H 2^-26 or 1 - (1 - H) 2^-26 to within
+// E 2^-26.  Near 1 the doubles are 2^-53 apart, so those bounds are widened by 2^-52.  H has the
+// same ambiguity at its own scale: within 2E of 0 or 1 (x within about 2^-55 M of 0 or M) the key
+// stays unknown and the order is exact.  Refined keys lie near 0 or 1, so two keys that do not
+// overlap may belong to x near 0 and y near M; the difference certificates of lane_cmp_exact are
+// therefore only used for overlapping keys.
+template <int N>
+GRNS_D void refine_key(const DevCtx *c, const int *__restrict__ x, long long smod, long long snum, MaxCand &e) {
+    if (e.idx < 0 || e.lo >= 0) return;
+    const int *p = x + e.idx * snum;
+    double A = 0.0, B = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < N; i++) {
+        const double dx = (double)p[(long long)i * smod];
+        A += fract_d(dx * c->shi[i]);
+        B = fma(dx, c->slo[i], B);
+    }
+    const double H = fract_d(A + B), E = c->err;
+    const double s = 1.0 / (double)(1 << KEY_SHIFT);
+    // H itself wraps: within 2E of 0 or 1 it cannot tell x < 2E 2^-26 M from x > (1 - 2E 2^-26) M
+    if (H > 2.0 * E && H < 0.25) {
+        e.lo = fmax(0.0, (H - E) * s);
+        e.hi = (H + E) * s;
+    } else if (H > 0.75 && H < 1.0 - 2.0 * E) {
+        e.lo = 1.0 - ((1.0 - H) + E) * s - 0x1p-52;
+        e.hi = fmin(1.0, 1.0 - ((1.0 - H) - E) * s + 0x1p-52);
+    }
+}
+
 constexpr int LANE_UNROLL = 1;  // the stage-1 loop keeps its occupancy: overlaps there are rare
 
 // Every lane proposes (e -> best): keys decide, then the exact order (lowest index among equal
 // values).  All 64 lanes of the wave must call this.
 template <int N>
 GRNS_D void max_merge(const DevCtx *c, const int *__restrict__ x, long long smod, long long snum, MaxCand &best,
-                      const MaxCand &e, bool valid) {
+                      const MaxCand &e_in, bool valid) {
+    MaxCand e = e_in;
     int d = valid ? max_decide(e, best) : 0;
+    if (d == 2 && (e.lo < 0 || best.lo < 0)) {  // rare: a key near 0 or M
+        refine_key<N>(c, x, smod, snum, e);
+        refine_key<N>(c, x, smod, snum, best);
+        d = max_decide(e, best);
+    }
     int r = 0;
     if (d == 2)
         r = lane_cmp_exact<LANE_UNROLL>(c, x + e.idx * snum, smod, x + best.idx * snum, smod,
@@ -959,6 +998,8 @@ GRNS_D MaxCand max_key(const DevCtx *c, const int (&xr)[N], long long j) {
     }
     return e;
 }
+
+
 
 template <int N>
 GRNS_D void block_max_reduce(const DevCtx *c, const int *__restrict__ x, long long smod, long long snum,
@@ -1001,7 +1042,10 @@ __global__ __launch_bounds__(BLOCK) void k_max_partial(const DevCtx *__restrict_
         max_merge<N>(c, x, smod, snum, best, e, valid);
     }
     block_max_reduce<N>(c, x, smod, snum, best, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = best;
+    if (threadIdx.x == 0) {
+        refine_key<N>(c, x, smod, snum, best);  // stage 2 orders by keys where it can
+        part[blockIdx.x] = best;
+    }
     // a copy of the winner's residues, so that stage 2 reads them from one contiguous line
     if (threadIdx.x < (unsigned)N && best.idx >= 0)
         part_res[(long long)blockIdx.x * N + threadIdx.x] = x[best.idx * snum + (long long)threadIdx.x * smod];
@@ -1016,6 +1060,7 @@ GRNS_D bool pivot_before(const MaxCand &a, const MaxCand &b) {
 
 constexpr int FBLOCK = 1024;             // threads of the stage-2 block
 constexpr int MAX_FINAL_PER_THREAD = 8;  // stage 2 takes at most FBLOCK * 8 candidates
+constexpr int MAX_PIVOT_ROUNDS = 3;      // then a tree over the live candidates (bounded depth)
 
 // Stage 2 (one block): the exact maximum of the per-block candidates, by rounds.  The live
 // candidate with the largest lower key bound is the pivot; every other live candidate is ordered
@@ -1024,7 +1069,12 @@ constexpr int MAX_FINAL_PER_THREAD = 8;  // stage 2 takes at most FBLOCK * 8 can
 // the tie index); those above stay live for the next round.  When none is above, the pivot's value
 // is the maximum and the tie index is the lowest index holding it; when exactly one is above, that
 // candidate is the maximum (every other candidate is at most the pivot, and the pivots of earlier
-// rounds are lower still) and holds its value alone.
+// rounds are lower still) and holds its value alone.  Rounds settle keys that separate the
+// candidates; when the keys do not (values within 2^-29 M of 0 or M carry no key, and every
+// candidate may survive a round but the pivot), after MAX_PIVOT_ROUNDS rounds the live candidates
+// are reduced by a tree of exact comparisons instead, log2(FBLOCK) levels over at most
+// MAX_FINAL_PER_THREAD candidates per thread.  Every live candidate is above all earlier pivots, so
+// the maximum and every candidate equal to it are still live.
 template <int N>
 __global__ __launch_bounds__(FBLOCK) void k_max_final(const DevCtx *__restrict__ c, const int *__restrict__ x,
                                                       long long smod, long long snum,
@@ -1044,7 +1094,7 @@ __global__ __launch_bounds__(FBLOCK) void k_max_final(const DevCtx *__restrict__
         if (k < nparts && part[k].idx >= 0) live |= 1u << t;
     }
     long long win = -1;
-    for (;;) {
+    for (int round = 0; round < MAX_PIVOT_ROUNDS; round++) {
         MaxCand b;
         b.idx = -1; b.lo = -2.0; b.hi = -2.0;
         int bk = -1;
@@ -1079,7 +1129,8 @@ __global__ __launch_bounds__(FBLOCK) void k_max_final(const DevCtx *__restrict__
             e.idx = -1; e.lo = -1.0; e.hi = -1.0;
             if (mine) e = part[k];
             int r = -1;
-            if (mine && e.idx != P.idx && e.hi >= P.lo)
+            if (mine && e.idx != P.idx && e.lo > P.hi) r = 1;  // the keys order them
+            else if (mine && e.idx != P.idx && e.hi >= P.lo)  // overlapping keys: exact order
                 r = lane_cmp_exact<16>(c, part_res + (long long)k * N, 1, pres, 1, e.lo >= 0 && P.lo >= 0);
             coop_resolve<N>(c, x, smod, snum, e.idx, P.idx, r);
             if (mine) {
@@ -1108,6 +1159,28 @@ __global__ __launch_bounds__(FBLOCK) void k_max_final(const DevCtx *__restrict__
             break;
         }
         __syncthreads();  // s_tie, s_nabove and sh are rewritten by the next round
+    }
+    if (win < 0 && __syncthreads_or(live != 0)) {  // uniform: the rounds left candidates live
+        MaxCand best;
+        best.idx = -1; best.lo = -1.0; best.hi = -1.0;
+        for (int t = 0; t < per; t++) {  // uniform trip count: max_merge runs wave groups
+            const bool mine = live >> t & 1;
+            MaxCand e;
+            e.idx = -1; e.lo = -1.0; e.hi = -1.0;
+            if (mine) e = part[T + t * FBLOCK];
+            max_merge<N>(c, x, smod, snum, best, e, mine);
+        }
+        sh[T] = best;
+        __syncthreads();
+        for (int h = FBLOCK / 2; h >= 1; h >>= 1) {
+            MaxCand other = (T < h) ? sh[T + h] : best;
+            MaxCand mine = sh[T];
+            __syncthreads();
+            max_merge<N>(c, x, smod, snum, mine, other, T < h);
+            if (T < h) sh[T] = mine;
+            __syncthreads();
+        }
+        win = sh[0].idx;
     }
     if (T < N && win >= 0) out[T] = x[win * snum + (long long)T * smod];
     if (T == 0) *out_idx = win;
@@ -1276,6 +1349,7 @@ GRNS_D bool mrc_part_before(const long long *__restrict__ part_idx, const int *_
     const int *da = part_dig + (long long)a * N;
     const int *db = part_dig + (long long)b * N;
     constexpr int C = N < 16 ? N : 16;
+    static_assert(N <= 16 || N % 16 == 0, "chunks of 16 digits must cover every digit");
 #pragma unroll 1
     for (int k0 = N - C; k0 >= 0; k0 -= C) {
         int u[C], v[C];
@@ -1911,6 +1985,75 @@ GRNS_D void group_mpint(const DevCtx *__restrict__ c, const Group &g, const Mpin
     }
 }
 
+// mp_int_t arrays in the reference's own layout (src/types.cuh:45-49, array of structs): element j
+// at base + j * stride bytes, digits[n] at offset 0, sign at 4n, eval[2] at eoff (8-byte aligned).
+struct MpAos {
+    const unsigned char *x;
+    const unsigned char *y;
+    unsigned char *z;
+    unsigned char *ovf;
+    long long stride;
+    long long eoff;
+};
+
+GRNS_D er_t load_er8(const er_t *p) {  // 8-byte aligned (the struct's own alignment)
+    er_t r;
+    r.frac = p->frac;
+    r.exp = p->exp;
+    return r;
+}
+
+GRNS_D void store_er8(er_t *p, const er_t &v) {
+    p->frac = v.frac;
+    p->exp = v.exp;
+}
+
+// group_mpint on mp_int_t structs: the digits of one element are contiguous, so the lanes of a
+// group read one line.  Every lane reads its operands before any lane writes, so z may be x or y.
+template <int OP>
+GRNS_D void group_mpint_aos(const DevCtx *__restrict__ c, const Group &g, const MpAos &A, long long j) {
+    const int n = c->n;
+    const bool act = g.gl < g.L;
+    const int *xd = reinterpret_cast<const int *>(A.x + j * A.stride);
+    const int *yd = reinterpret_cast<const int *>(A.y + j * A.stride);
+    const er_t *xe = reinterpret_cast<const er_t *>(A.x + j * A.stride + A.eoff);
+    const er_t *ye = reinterpret_cast<const er_t *>(A.y + j * A.stride + A.eoff);
+    int a = act ? xd[g.gl] : 0, b = act ? yd[g.gl] : 0;
+    int sx = xd[n], sy = yd[n];
+    if (OP == 1) sy ^= 1;
+    er_t r0, r1;
+    int minus;
+    mpint_bounds<OP>(c, sx, sy, load_er8(xe), load_er8(xe + 1), load_er8(ye), load_er8(ye + 1), r0, r1, minus);
+    int r = 0;
+    if (act) {
+        int m = c->m[g.gl];
+        r = mpint_digit<OP>(a, b, sx, sy, m, c->dinv[g.gl]);
+        if (OP != 2 && minus) r = r == 0 ? 0 : (m - r);
+    }
+    int sr = (OP == 2) ? (group_all(g, !act || r == 0) ? 0 : (sx ^ sy)) : minus;
+    int *zd = reinterpret_cast<int *>(A.z + j * A.stride);
+    er_t *ze = reinterpret_cast<er_t *>(A.z + j * A.stride + A.eoff);
+    if (act) zd[g.gl] = r;
+    if (g.gl == 0) {
+        zd[n] = sr;
+        store_er8(ze, r0);
+        store_er8(ze + 1, r1);
+        if (A.ovf) A.ovf[j] = check_overflow(c, r0, r1);
+    }
+}
+
+template <int OP>
+__global__ __launch_bounds__(BLOCK) void k_mpint_aos(const DevCtx *__restrict__ c, int G, MpAos A, long long count) {
+    const Group g = make_group(G, c->n);
+    const int P = 64 / G;
+    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
+    for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
+        const long long j = j0 + (threadIdx.x & 63) / G;
+        if (j < count) group_mpint_aos<OP>(c, g, A, j);
+    }
+}
+
 template <int N, int OP>
 __global__ __launch_bounds__(BLOCK) void k_mpint(const DevCtx *__restrict__ c, MpintArgs A, long long count) {
     for (long long j = (long long)blockIdx.x * BLOCK + threadIdx.x; j < count; j += (long long)gridDim.x * BLOCK)
@@ -1940,6 +2083,47 @@ GRNS_D void group_mpint_div(const DevCtx *c, const Group &g, const MpintArgs &A,
         A.re[2 * j] = e.low;
         A.re[2 * j + 1] = e.upp;
     }
+}
+
+// group_mpint_div on mp_int_t structs (z may be x or y: every lane reads before any lane writes).
+template <int GD>
+GRNS_D void group_mpint_div_aos(const DevCtx *c, const Group &g, const MpAos &A, long long j, int &status) {
+    const int n = c->n;
+    const bool act = g.gl < g.L;
+    const int *xd = reinterpret_cast<const int *>(A.x + j * A.stride);
+    const int *yd = reinterpret_cast<const int *>(A.y + j * A.stride);
+    const int xi = act ? xd[g.gl] : 0, yi = act ? yd[g.gl] : 0;
+    const int xs = xd[n], ys = yd[n];
+    int q, r;
+    if (GD >= c->words && div_words_ok(c, g, xi, yi)) group_div_words<GD>(c, g, xi, yi, q, r);
+    else group_div<true>(c, g, xi, yi, q, r, status);
+    const bool rz = group_all(g, !act || r == 0);
+    if (xs == 1 && !rz && act) q = (int)rem_trunc((long long)q + 1, c->m[g.gl], c->dinv[g.gl]);
+    const interval_t e = group_eval<false>(c, g, q, status);
+    const bool qz = group_all(g, !act || q == 0);
+    int *zd = reinterpret_cast<int *>(A.z + j * A.stride);
+    er_t *ze = reinterpret_cast<er_t *>(A.z + j * A.stride + A.eoff);
+    if (act) zd[g.gl] = q;
+    if (g.gl == 0) {
+        zd[n] = qz ? 0 : (xs ^ ys);
+        store_er8(ze, e.low);
+        store_er8(ze + 1, e.upp);
+    }
+}
+
+template <int GD>
+__global__ __launch_bounds__(BLOCK) void k_mpint_div_aos(const DevCtx *__restrict__ c, MpAos A, long long count,
+                                                         int *__restrict__ status) {
+    const Group g = make_group(GD, c->n);
+    constexpr int P = 64 / GD;
+    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * BLOCK) >> 6;
+    int st = 0;
+    for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
+        const long long j = j0 + (threadIdx.x & 63) / GD;
+        if (j < count) group_mpint_div_aos<GD>(c, g, A, j, st);
+    }
+    if (g.gl == 0 && st) atomicOr(status, st);
 }
 
 template <int GD>

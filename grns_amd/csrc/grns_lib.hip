@@ -94,6 +94,19 @@ grns_status build_constants(grns_ctx *c, const int *moduli, int n) {
         kb &= ~((1ull << 31) - 1);  // keep 22 significant bits (chi is a normal double in (0, 1))
         std::memcpy(&h.khi[i], &kb, 8);
         h.klo[i] = (h.chi[i] - h.khi[i]) + h.clo[i];  // the difference is exact
+        {  // the same split for (2^KEY_SHIFT w_i mod m_i) / m_i
+            long long ws = h.w[i];
+            for (int k = 0; k < KEY_SHIFT; k++) ws = (ws * 2) % m;
+            double shi = (double)ws / (double)moduli[i];
+            double srem = std::fma(-shi, (double)moduli[i], (double)ws);
+            unsigned long long sb;
+            std::memcpy(&sb, &shi, 8);
+            sb &= ~((1ull << 31) - 1);
+            double sh;
+            std::memcpy(&sh, &sb, 8);
+            h.shi[i] = sh;
+            h.slo[i] = (shi - sh) + srem / (double)moduli[i];
+        }
         h.c32[i] = (int)((1ull << 32) % (unsigned long long)m);
     }
     for (int t = 0; t < THRESH; t++) {
@@ -563,9 +576,12 @@ grns_status grns_cmp(grns_ctx *c, const int *x, long long xsm, long long xsn, co
 }
 
 grns_status grns_div(grns_ctx *c, const int *x, const int *d, long long smod, long long snum, size_t count, int *q,
-                     int *r, int fast, void *stream) {
+                     int *r, int flags, void *stream) {
     if (c && c->device < 0) return GRNS_ERR_ARG;
     if (!c || (count && (!x || !d || !q || !r))) return GRNS_ERR_ARG;
+    if (flags & ~(GRNS_DIV_FAST | GRNS_DIV_REFERENCE)) return GRNS_ERR_ARG;
+    const bool fast = (flags & GRNS_DIV_FAST) != 0;
+    const bool reference = (flags & GRNS_DIV_REFERENCE) != 0 || c->reference_loops;
     if (count == 0) return GRNS_OK;
     if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
     DeviceGuard guard(c->device);
@@ -573,7 +589,7 @@ grns_status grns_div(grns_ctx *c, const int *x, const int *d, long long smod, lo
     dim3 grid((unsigned)grid_groups(count, G)), blk(BLOCK);
     long long cnt = (long long)count;
     // canonical pairs with d != 0: word division; the reference loop for the others (rare pass)
-    const int GW = c->reference_loops ? 0 : words_group(c);
+    const int GW = reference ? 0 : words_group(c);
     if (GW) {
         RareQ rq;
         grns_status qs = rare_queue(c, stream, rq);
@@ -705,6 +721,10 @@ grns_status grns_scale2pow(grns_ctx *c, const int *x, long long smod, long long 
     hipLaunchKernelGGL(k_scale2pow, grid, blk, 0, S(stream), c->d, G, x, smod, snum, D, cnt, y);
     return launch_check();
 }
+
+// Offset of eval[2] in the reference's mp_int_t (int digits[n]; int sign; er_float_t eval[2];):
+// after the 4n + 4 bytes of digits and sign, at the double's 8-byte alignment.
+static size_t mpint_aos_eoff(int n) { return ((size_t)4 * n + 4 + 7) / 8 * 8; }
 
 // Stage 1 of the arg-max runs one resident wave of blocks (CUs x blocks per CU, queried at context
 // creation), each over a contiguous chunk that is a multiple of the block size.
@@ -897,6 +917,54 @@ grns_status grns_mpint_div(grns_ctx *c, const grns_mpint_array *x, const grns_mp
         case 16: hipLaunchKernelGGL((k_mpint_div<16>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
         case 32: hipLaunchKernelGGL((k_mpint_div<32>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
         default: hipLaunchKernelGGL((k_mpint_div<64>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+    }
+    return launch_check();
+}
+
+size_t grns_mpint_aos_bytes(const grns_ctx *c) {
+    if (!c) return 0;
+    return mpint_aos_eoff(c->n) + 2 * sizeof(grns_er);
+}
+
+grns_status grns_mpint_aos(grns_ctx *c, int op, const void *x, const void *y, void *z, size_t count, size_t elem_bytes,
+                           unsigned char *ovf, void *stream) {
+    if (!c || c->device < 0) return GRNS_ERR_ARG;
+    if (op != GRNS_OP_MPINT_ADD && op != GRNS_OP_MPINT_SUB && op != GRNS_OP_MPINT_MUL && op != GRNS_OP_MPINT_DIV)
+        return GRNS_ERR_ARG;
+    if (elem_bytes != grns_mpint_aos_bytes(c)) return GRNS_ERR_ARG;  // a different RNS_MODULI_SIZE
+    if (count == 0) return GRNS_OK;
+    if (!x || !y || !z) return GRNS_ERR_ARG;
+    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
+    DeviceGuard guard(c->device);
+    MpAos A;
+    A.x = static_cast<const unsigned char *>(x);
+    A.y = static_cast<const unsigned char *>(y);
+    A.z = static_cast<unsigned char *>(z);
+    A.ovf = op == GRNS_OP_MPINT_DIV ? nullptr : ovf;  // the reference checks no overflow for a division
+    A.stride = (long long)elem_bytes;
+    A.eoff = (long long)mpint_aos_eoff(c->n);
+    const long long cnt = (long long)count;
+    dim3 blk(BLOCK);
+    if (op == GRNS_OP_MPINT_DIV) {
+        const int GW = words_group(c) ? words_group(c) : groups_for(c->n);
+        dim3 grid((unsigned)grid_groups(count, GW));
+        switch (GW) {
+            case 1: hipLaunchKernelGGL((k_mpint_div_aos<1>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+            case 2: hipLaunchKernelGGL((k_mpint_div_aos<2>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+            case 4: hipLaunchKernelGGL((k_mpint_div_aos<4>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+            case 8: hipLaunchKernelGGL((k_mpint_div_aos<8>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+            case 16: hipLaunchKernelGGL((k_mpint_div_aos<16>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+            case 32: hipLaunchKernelGGL((k_mpint_div_aos<32>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+            default: hipLaunchKernelGGL((k_mpint_div_aos<64>), grid, blk, 0, S(stream), c->d, A, cnt, c->d_status); break;
+        }
+        return launch_check();
+    }
+    const int G = groups_for(c->n);
+    dim3 grid((unsigned)grid_groups(count, G));
+    switch (op) {
+        case GRNS_OP_MPINT_ADD: hipLaunchKernelGGL((k_mpint_aos<0>), grid, blk, 0, S(stream), c->d, G, A, cnt); break;
+        case GRNS_OP_MPINT_SUB: hipLaunchKernelGGL((k_mpint_aos<1>), grid, blk, 0, S(stream), c->d, G, A, cnt); break;
+        default: hipLaunchKernelGGL((k_mpint_aos<2>), grid, blk, 0, S(stream), c->d, G, A, cnt); break;
     }
     return launch_check();
 }

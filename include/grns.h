@@ -75,10 +75,21 @@ grns_status grns_eval(grns_ctx *ctx, const int *x, long long mod_stride, long lo
 grns_status grns_cmp(grns_ctx *ctx, const int *x, long long x_mod_stride, long long x_num_stride, const int *y,
                      long long y_mod_stride, long long y_num_stride, size_t count, int *sign, void *stream);
 
-/* rns_div (fast = 0, src/rnsdiv.cuh:19-61) or rns_div_fast (fast = 1, :67-114): q, r with the
- * layout of x (x, d, q, r share mod_stride / num_stride). */
+/* rns_div (src/rnsdiv.cuh:19-61) or rns_div_fast (:67-114): q, r with the layout of x (x, d, q, r
+ * share mod_stride / num_stride).  `flags` (0 and 1 keep their old meaning of `fast`):
+ *   GRNS_DIV_FAST       rns_div_fast rather than rns_div;
+ *   GRNS_DIV_REFERENCE  the reference's loop for every pair, so that every result is the CPU path's
+ *                       bit for bit.  Without it, pairs of canonical residues with d != 0 are divided
+ *                       on their binary forms: the exact quotient and remainder, which is also the
+ *                       CPU path's result except on the pairs where the reference's reciprocal
+ *                       reduction wraps int32 (src/modular.cuh:74-79) and its loop returns another
+ *                       number (about one pair in 10^6 on uniform operands); all other pairs (and
+ *                       the pairs with d = 0 or non-canonical residues) run the reference loop.
+ * Other bits: GRNS_ERR_ARG. */
+#define GRNS_DIV_FAST 1
+#define GRNS_DIV_REFERENCE 2
 grns_status grns_div(grns_ctx *ctx, const int *x, const int *d, long long mod_stride, long long num_stride,
-                     size_t count, int *q, int *r, int fast, void *stream);
+                     size_t count, int *q, int *r, int flags, void *stream);
 
 /* rns_scale2pow (src/rnsscal.cuh:93-119): y_j = x_j scaled by 2^D[j]. */
 grns_status grns_scale2pow(grns_ctx *ctx, const int *x, long long mod_stride, long long num_stride,
@@ -155,6 +166,17 @@ grns_status grns_mpint_mul(grns_ctx *ctx, const grns_mpint_array *x, const grns_
  * fields of x and y are not read, and overflow (may be NULL) is not written. */
 grns_status grns_mpint_div(grns_ctx *ctx, const grns_mpint_array *x, const grns_mpint_array *y,
                            const grns_mpint_array *z, size_t count, unsigned char *overflow, void *stream);
+
+/* mpint_add / mpint_sub / mpint_mul / mpint_div (src/mpint.cuh:160-296) over arrays of the
+ * reference's own mp_int_t (src/types.cuh:45-49: int digits[n]; int sign; er_float_t eval[2]; element
+ * j at (char *)x + j * elem_bytes), for code that keeps that layout (the drop-in header's
+ * cuda::mpint_*_array).  op: GRNS_OP_MPINT_ADD, _SUB, _MUL (overflow as in grns_mpint_add) or
+ * GRNS_OP_MPINT_DIV (overflow not written).  elem_bytes must be grns_mpint_aos_bytes(ctx) =
+ * sizeof(mp_int_t) for the context's n.  z may be x or y.  Results are those of the array forms. */
+#define GRNS_OP_MPINT_DIV 10
+size_t grns_mpint_aos_bytes(const grns_ctx *ctx);
+grns_status grns_mpint_aos(grns_ctx *ctx, int op, const void *x, const void *y, void *z, size_t count,
+                           size_t elem_bytes, unsigned char *overflow, void *stream);
 
 /* ---------------------------------------------------------------- batched form (no reference
  * counterpart): many arrays of uneven length and several operations of one moduli set fused into

@@ -30,9 +30,9 @@ double grns_debug_filter_err(const grns_ctx *ctx);
  * kernel time in ms and the launch count since the last read, then clears them.  Not thread-safe. */
 grns_status grns_debug_stage_timing(grns_ctx *ctx, int enable);
 grns_status grns_debug_stage_read(grns_ctx *ctx, int op, double *ms_total, long long *launches);
-/* Test hook: on != 0 makes grns_div and grns_scale2pow run the reference's loops
- * (src/rnsdiv.cuh, src/rnsscal.cuh) for every number instead of the word forms, so that the two
- * can be compared on the device. */
+/* Test hook: on != 0 makes grns_div (as its GRNS_DIV_REFERENCE flag does) and grns_scale2pow run
+ * the reference's loops (src/rnsdiv.cuh, src/rnsscal.cuh) for every number instead of the word
+ * forms, so that the two can be compared on the device. */
 grns_status grns_debug_reference_loops(grns_ctx *ctx, int on);
 /* Scheduler test hook: a server whose launches run nothing and complete delay_us after issue;
  * it accepts host-only contexts (device -1), so admission and batching run without a GPU. */

@@ -181,6 +181,43 @@ def test_div_words_against_reference_loop(ctxs, dev, n, count, fast):
     assert c.status() == 0
 
 
+def test_div_full_size_exact(ctxs, dev):
+    """The division configuration at full size (64 moduli, 2^20 pairs, dividends uniform in [0, M),
+    divisor bit lengths uniform in [1, 1980]): every pair satisfies x = q d + r modulo every m_i
+    (int64 on the device); on a seeded sample of 4096 pairs, q and r are the exact quotient and
+    remainder (0 <= r < d by CRT) and equal the oracle's, except pairs on which the reference's loop
+    itself wraps int32 (counted and printed); with GRNS_DIV_REFERENCE the sample equals the oracle
+    bit for bit, wrapped pairs included."""
+    import grns_amd as G
+    from grns_amd.workload import Synth
+    c, o, mod = ctxs[64]
+    M = G.product(mod)
+    count = 1 << 20
+    x, d, _, _ = Synth(mod).div(count, seed=808, max_bits=1980, max_shift=1900)
+    q, r = c.div(x, d, fast=True)
+    m = torch.tensor(mod, dtype=torch.int64, device=x.device).view(-1, 1)
+    lhs = (q.long() * d.long() + r.long() - x.long()) % m
+    assert int(lhs.abs().sum()) == 0, "x != q d + r modulo some m_i"
+    rng = np.random.default_rng(809)
+    pos = np.sort(rng.choice(count, size=4096, replace=False))
+    ix = torch.from_numpy(pos).to(x.device)
+    xs, ds = x.index_select(1, ix).cpu().numpy(), d.index_select(1, ix).cpu().numpy()
+    qs, rs = q.index_select(1, ix).cpu().numpy(), r.index_select(1, ix).cpu().numpy()
+    for j in range(len(pos)):
+        xv, dv = _crt(xs[:, j], mod, M), _crt(ds[:, j], mod, M)
+        qv, rv = _crt(qs[:, j], mod, M), _crt(rs[:, j], mod, M)
+        assert (qv, rv) == divmod(xv, dv) and 0 <= rv < dv
+    oq, orr = o.div_array(xs, ds, fast=True)
+    differ = np.nonzero((oq != qs).any(axis=0) | (orr != rs).any(axis=0))[0]
+    for j in differ:  # the oracle (the CPU path) is not the exact quotient there
+        xv, dv = _crt(xs[:, j], mod, M), _crt(ds[:, j], mod, M)
+        assert (_crt(oq[:, j], mod, M), _crt(orr[:, j], mod, M)) != divmod(xv, dv)
+    print(f"division n=64, 2^20 pairs: sample of 4096, {len(differ)} differ from the CPU path (int32 wrap)")
+    qr, rr = c.div(x.index_select(1, ix).contiguous(), d.index_select(1, ix).contiguous(), fast=True, reference=True)
+    assert (qr.cpu().numpy() == oq).all() and (rr.cpu().numpy() == orr).all()
+    assert c.status() == 0
+
+
 def test_div_rare_queue_overflow(ctxs, dev):
     """More pairs for the reference loop than the rare queue holds (negative, non-canonical residues
     in every other pair): the overflow is marked and the rare pass still divides every pair."""
@@ -341,6 +378,49 @@ def test_max_noncanonical_residues(ctxs, dev, n):
         res, idx = c.max(_dev(x, dev), mrc=mrc)
         assert int(idx.item()) == truth, mrc
         assert res.cpu().numpy().tolist() == x[:, truth].tolist(), mrc
+
+
+@pytest.mark.parametrize("n", [8, 32, 64])
+@pytest.mark.parametrize("case", ["ascending", "near_M", "random_small"])
+def test_max_small_magnitudes(ctxs, dev, n, case):
+    """2^20 numbers whose values lie within 2^-40 M of 0 or of M, where the arg-max keys carry no
+    order: x_j = j (ascending), x_j = M - 1 - j (the maximum is the first), and random values of at
+    most log2(M) - 41 bits with a planted maximum 2^(log2 M - 40) and 100 near-ties one below it.
+    Index and residues against the integers, both pipelines; stage 2 of the key pipeline stays
+    bounded (a few pivot rounds, then a tree of exact comparisons)."""
+    import time
+    c, o, mod = ctxs[n]
+    from grns_amd.moduli import product
+    from grns_amd.workload import Synth
+    M = product(mod)
+    count = 1 << 20
+    j = np.arange(count, dtype=np.int64)
+    if case == "ascending":
+        x = _dev(np.stack([j % m for m in mod]).astype(np.int32), dev)
+        truth, value = count - 1, count - 1
+    elif case == "near_M":
+        x = _dev(np.stack([(m - 1 - j) % m for m in mod]).astype(np.int32), dev)
+        truth, value = 0, M - 1
+    else:
+        syn = Synth(mod)
+        L = M.bit_length() - 1
+        bits = torch.randint(1, L - 41, (count,), generator=torch.Generator().manual_seed(n), dtype=torch.int32)
+        x = syn.bits(bits.to(dev), seed=7 + n, top_set=False)
+        rnd = random.Random(n)
+        pos = rnd.sample(range(count), 101)
+        value = (1 << (L - 40)) + rnd.randrange(1 << 20)
+        syn.place(x, [value] + [value - 1] * 100, pos)
+        truth = pos[0]
+    want = [value % m for m in mod]
+    for mrc in (False, True):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res, idx = c.max(x, mrc=mrc)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        assert int(idx.item()) == truth, (case, mrc)
+        assert res.cpu().numpy().tolist() == want, (case, mrc)
+        print(f"max small magnitudes n={n} {case} mrc={mrc}: {dt * 1e3:.1f} ms")
 
 
 @pytest.mark.parametrize("n", [8, 32])
@@ -707,3 +787,80 @@ def test_other_moduli_counts(dev, n):
     finally:
         c.close()
         o.close()
+
+
+SHIPPED_SETS = ["src/params.h", "src/params/moduli-64-4096-bit/params.16m.256b.h",
+                "src/params/moduli-64-4096-bit/params.8m.128b.h", "src/params/moduli-10-69-bit/params.10m.36b.h",
+                "src/params/moduli-10-69-bit/params.16m.69b.h", "src/params/moduli-100-6400-bit/params.32m.800b.h",
+                "src/params/moduli-100-6400-bit/params.64m.1600b.h"]
+
+
+@pytest.mark.parametrize("name", SHIPPED_SETS)
+def test_shipped_moduli_sets(dev, name):
+    """The moduli sets GRNS ships as params.h files (tests/golden/params_sets.json): pairwise coprime
+    but not prime, from 3..31 up to ~2^31, so that every modulus-size-dependent path (reciprocal
+    reductions of small moduli, the key and certificate bounds, the word division with few words)
+    runs.  Every operation against the oracle bit for bit; arg-max against the integers."""
+    import json
+    import os
+
+    import grns_amd as G
+    from oracle.oracle import Oracle
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "params_sets.json")) as f:
+        mod = {e["file"]: e["moduli"] for e in json.load(f)}[name]
+    M = G.product(mod)
+    c, o = G.Context(mod, 0), Oracle(mod)
+    rnd = random.Random(len(mod) * 1000 + mod[0] % 1000)
+    vals = eval_inputs(mod, 3000, seed=3)
+    x = to_res(vals, mod)
+    xd = _dev(x, dev)
+    for fast in (False, True):
+        assert c.eval(xd, fast=fast).cpu().numpy().tobytes() == o.eval_array(x, fast=fast).tobytes(), fast
+    xs, ys = cmp_pairs(mod, 3000, seed=4)
+    a, b = to_res(xs, mod), to_res(ys, mod)
+    assert (c.cmp(_dev(a, dev), _dev(b, dev)).cpu().numpy() == o.cmp_array(a, b)).all()
+    dx, dd = div_inputs(mod, 400, seed=5)
+    p, q = to_res(dx, mod), to_res(dd, mod)
+    for fast in (True, False):
+        gq, gr = c.div(_dev(p, dev), _dev(q, dev), fast=fast)
+        oq, orr = o.div_array(p, q, fast=fast)
+        assert (gq.cpu().numpy() == oq).all() and (gr.cpu().numpy() == orr).all(), fast
+    D = np.array([rnd.randrange(0, o.log2M + 3) for _ in vals], np.int32)
+    assert (c.scale2pow(xd, _dev(D, dev)).cpu().numpy() == o.scale2pow_array(x, D.astype(np.uint32))).all()
+    assert (c.mrc(xd).cpu().numpy() == np.stack([o.mrc(x[:, j]) for j in range(x.shape[1])], axis=1)).all()
+    truth = max(range(len(vals)), key=lambda k: (vals[k], -k))
+    for mrc in (False, True):
+        if len(mod) not in (4, 8, 16, 32, 64):  # the arg-max kernels are built for these sizes
+            with pytest.raises(G.GrnsError) as ei:
+                c.max(xd, mrc=mrc)
+            assert ei.value.code == G.GRNS_ERR_UNSUPPORTED
+            continue
+        res, idx = c.max(xd, mrc=mrc)
+        assert int(idx.item()) == truth and res.cpu().numpy().tolist() == x[:, truth].tolist(), mrc
+    if M > 1 << 40:
+        bits = (M.bit_length() - 6) // 2
+        mv = [rnd.choice([-1, 1]) * rnd.randrange(1 << rnd.randrange(1, bits)) for _ in range(600)]
+        nv = [rnd.choice([-1, 1]) * rnd.randrange(1 << rnd.randrange(1, bits)) for _ in range(600)]
+
+        def side(v):
+            dg = to_res([abs(t) for t in v], mod)
+            sg = np.array([1 if t < 0 else 0 for t in v], np.int32)
+            ev = o.eval_array(dg)
+            e = np.zeros((len(v), 2), ev["low"].dtype)
+            e[:, 0], e[:, 1] = ev["low"], ev["upp"]
+            return (dg, sg, e), (_dev(dg, dev), _dev(sg, dev), _dev(e.view(np.float64).reshape(len(v), 2, 2), dev))
+        (Xn, X), (Yn, Y) = side(mv), side(nv)
+        for op in ("add", "sub", "mul"):
+            Z = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
+            c.mpint(op, X, Y, Z, len(mv))
+            rd, rs, re, _ = o.mpint_array(op, *Xn, *Yn)
+            assert (Z[0].cpu().numpy() == rd).all() and (Z[1].cpu().numpy() == rs).all(), op
+            assert Z[2].cpu().numpy().tobytes() == re.tobytes(), op
+    # the status bit for a refinement factor outside RNS_POW2 (the reference reads past its table)
+    # is raised exactly when the oracle met such an input (it returns NaN bounds there)
+    ev = [o.eval_array(x, fast=f) for f in (False, True)]
+    past_table = any(np.isnan(e["low"]["frac"]).any() or np.isnan(e["upp"]["frac"]).any() for e in ev)
+    assert c.status() == (G.STATUS_REFINE_RANGE if past_table else 0)
+    c.close()
+    o.close()

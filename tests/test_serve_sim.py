@@ -35,6 +35,9 @@ def _task(rnd):
     t.op = rnd.choice([0, 1, 2, 3, 5, 7, 8, 9])
     t.count = rnd.choice([1, 17, 256, 1000, 3000])
     t.a = t.b = t.out0 = t.out1 = t.D = 8  # never dereferenced in simulation
+    for m in (t.mx, t.my, t.mz):
+        m.digits = m.sign = m.eval = 8
+        m.ld = t.count
     return t
 
 
@@ -137,4 +140,35 @@ def test_sim_bad_requests(ctxs):
     mx = G.BatchTask()
     mx.op, mx.count = 9, 5  # arg-max without its output pointers
     assert _submit(L, h, ctxs[8], [mx])[0] == 1
+    # fused operations with a missing operand or output, and mpint arrays shorter than the task
+    for op in range(0, 9):
+        t = _task(random.Random(op))
+        t.op, t.count = op, 100
+        for m in (t.mx, t.my, t.mz):
+            m.ld = 100
+        assert _submit(L, h, ctxs[8], [t])[0] == 0
+        for field in ("a", "b", "out0", "out1", "D"):
+            u = _task(random.Random(op))
+            u.op, u.count = op, 100
+            for m in (u.mx, u.my, u.mz):
+                m.ld = 100
+            setattr(u, field, None)
+            needed = {0: "a out0", 1: "a out0", 2: "a b out0", 6: "a b out0 out1", 7: "a b out0 out1",
+                      8: "a D out0"}.get(op, "")
+            assert _submit(L, h, ctxs[8], [u])[0] == (1 if field in needed.split() else 0), (op, field)
+        if 3 <= op <= 5:
+            for arr in ("mx", "my", "mz"):
+                for field in ("digits", "sign", "eval"):
+                    u = _task(random.Random(op))
+                    u.op, u.count = op, 100
+                    for m in (u.mx, u.my, u.mz):
+                        m.ld = 100
+                    setattr(getattr(u, arr), field, None)
+                    assert _submit(L, h, ctxs[8], [u])[0] == 1, (op, arr, field)
+            u = _task(random.Random(op))
+            u.op, u.count = op, 100
+            for m in (u.mx, u.my, u.mz):
+                m.ld = 99  # shorter than the task
+            assert _submit(L, h, ctxs[8], [u])[0] == 1
+    L.grns_serve_drain(h)
     assert L.grns_serve_destroy(h) == 0
