@@ -51,14 +51,15 @@ def product(moduli) -> int:
     return m
 
 
-def params_header(moduli, threshold: int = 30, rel_error: str = "0.0000001") -> str:
+def params_header(moduli, threshold: int = 30, rel_error: str = "0.0000001", note: str | None = None) -> str:
     """Text of a params.h for ``moduli`` with the macro names of src/params.h:11-39."""
     n = len(moduli)
     log2m = product(moduli).bit_length() - 1
     vals = ",".join(str(v) for v in moduli)
+    note = note or f"{n} largest primes below 2^31, ascending"
     return f"""/*
  *  The set of RNS moduli and other parameters of GRNS
- *  ({n} largest primes below 2^31, ascending; floor(log2 M) = {log2m})
+ *  ({note}; floor(log2 M) = {log2m})
  */
 
 #ifndef GRNS_PARAMS_H
