@@ -182,10 +182,49 @@ def traffic_from_counters(fetch_dir, write_dir, out_path):
     log(f"[bench] wrote {out_path}")
 
 
-def cpu_per_op(threads, budget=2.0):
+CPU_RUNS = 7  # timed runs per CPU figure (after one untimed run); the median is reported
+
+
+def median_rate(fn, count, runs=CPU_RUNS):
+    """Median over `runs` timed calls of fn (after one untimed call) of count / seconds, and the runs."""
+    import statistics
+    fn()
+    rates = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        rates.append(count / (time.perf_counter() - t0))
+    return statistics.median(rates), rates
+
+
+def cpu_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+
+
+def cpu_headline(mod, syn, count):
+    """The reference's arg-max pipeline on the CPU (the oracle: evaluate every number, then the
+    rns_max_cmp reduction) over a sample of the headline workload: median of CPU_RUNS runs on all
+    host threads (the baseline's value) and on one thread."""
+    from oracle.oracle import Oracle
+    orc = Oracle(mod)
+    threads = cpu_threads()
+    out = {}
+    for nt, sample in ((threads, min(count, 1 << 22)), (1, min(count, 1 << 19))):
+        xh = syn.max_planted(sample, seed=2)[0].cpu().numpy()
+        med, rates = median_rate(lambda: orc.max_eval_array(xh, nthreads=nt), sample)
+        out[nt] = {"value": round(med), "unit": "RNS ops/s", "cores": nt, "runs": [round(r) for r in rates],
+                   "sample": f"rns_max pipeline (evaluate + rns_max_cmp reduction, CPU path) over 2^"
+                             f"{sample.bit_length() - 1} numbers of the same workload, median of {CPU_RUNS} runs, "
+                             f"{nt} OpenMP thread{'s' if nt > 1 else ''}"}
+    base = dict(out[threads], kind="port")
+    base["one_core"] = out[1]
+    return base
+
+
+def cpu_per_op(threads):
     """The oracle (the reference's CPU path restated) on bounded samples of the eval/cmp (n = 32),
-    division/scaling (n = 64) and mpint mul (n = 32) workloads, each repeated for about `budget`
-    seconds on `threads` OpenMP threads: operations per second."""
+    division/scaling (n = 64) and mpint mul (n = 32) workloads: operations per second, median of
+    CPU_RUNS runs, on `threads` OpenMP threads and on one."""
     import numpy as np
     import torch
 
@@ -193,40 +232,40 @@ def cpu_per_op(threads, budget=2.0):
     from grns_amd.workload import Synth
     from oracle.oracle import ER_DTYPE, Oracle
 
-    def rate(fn, count):
-        passes, t0 = 0, time.perf_counter()
-        while True:
-            fn()
-            passes += 1
-            dt = time.perf_counter() - t0
-            if dt >= budget:
-                return passes * count / dt, passes, dt
-
-    out = {}
     mod32, mod64 = G.largest_primes_below(32), G.largest_primes_below(64)
     o32, o64 = Oracle(mod32), Oracle(mod64)
     s32 = Synth(mod32)
     xe, ye, _ = s32.eval_cmp(1 << 18, seed=1)
     xh, yh = xe.cpu().numpy(), ye.cpu().numpy()
-    for name, fn in (("eval", lambda: o32.eval_array(xh, nthreads=threads)),
-                     ("cmp", lambda: o32.cmp_array(xh, yh, nthreads=threads))):
-        v, p, dt = rate(fn, xh.shape[1])
-        out[name] = {"value": round(v), "unit": "ops/s", "cores": threads, "sample": f"2^18 of the n=32 workload x {p}, {dt:.1f}s"}
     x, d, D, _ = Synth(mod64).div(1 << 12, seed=3, max_bits=1980, max_shift=1900)
-    xh, dh, Dh = x.cpu().numpy(), d.cpu().numpy(), D.cpu().numpy().view(np.uint32)
-    for name, fn in (("div_fast", lambda: o64.div_array(xh, dh, fast=True, nthreads=threads)),
-                     ("scale2pow", lambda: o64.scale2pow_array(xh, Dh, nthreads=threads))):
-        v, p, dt = rate(fn, xh.shape[1])
-        out[name] = {"value": round(v), "unit": "ops/s", "cores": threads, "sample": f"2^12 of the n=64 workload x {p}, {dt:.1f}s"}
+    dxh, ddh, Dh = x.cpu().numpy(), d.cpu().numpy(), D.cpu().numpy().view(np.uint32)
     c32 = G.Context(mod32, 0)
     bits = torch.full((1 << 18,), 480, dtype=torch.int32, device="cuda")
     X = [s32.bits(bits, 5, k, top_set=False) for k in (1, 2)]
     E = [np.ascontiguousarray(c32.eval(t).cpu().numpy()).view(ER_DTYPE).reshape(-1, 2) for t in X]
     sg = np.zeros(1 << 18, np.int32)
     xa, ya = X[0].cpu().numpy(), X[1].cpu().numpy()
-    v, p, dt = rate(lambda: o32.mpint_array("mul", xa, sg, E[0], ya, sg, E[1], nthreads=threads), 1 << 18)
-    out["mpint_mul"] = {"value": round(v), "unit": "ops/s", "cores": threads,
-                        "sample": f"2^18 of the n=32 workload x {p}, {dt:.1f}s"}
+
+    def cols(a, k):
+        return np.ascontiguousarray(a[:, :k])
+
+    # (name, sample at all threads, sample at one thread, function of (sample, threads), workload)
+    ops = (("eval", 1 << 18, 1 << 15, lambda k, t: o32.eval_array(cols(xh, k), nthreads=t), "n=32"),
+           ("cmp", 1 << 18, 1 << 14, lambda k, t: o32.cmp_array(cols(xh, k), cols(yh, k), nthreads=t), "n=32"),
+           ("div_fast", 1 << 12, 1 << 8, lambda k, t: o64.div_array(cols(dxh, k), cols(ddh, k), fast=True, nthreads=t),
+            "n=64"),
+           ("scale2pow", 1 << 12, 1 << 9, lambda k, t: o64.scale2pow_array(cols(dxh, k), Dh[:k], nthreads=t), "n=64"),
+           ("mpint_mul", 1 << 18, 1 << 16,
+            lambda k, t: o32.mpint_array("mul", cols(xa, k), sg[:k], E[0][:k], cols(ya, k), sg[:k], E[1][:k],
+                                         nthreads=t), "n=32"))
+    out = {}
+    for name, k_all, k_one, fn, wl in ops:
+        rec = {}
+        for nt, k in ((threads, k_all), (1, k_one)):
+            med, _ = median_rate(lambda: fn(k, nt), k)
+            rec[nt] = {"value": round(med), "unit": "ops/s", "cores": nt,
+                       "sample": f"2^{k.bit_length() - 1} of the {wl} workload, median of {CPU_RUNS} runs"}
+        out[name] = dict(rec[threads], one_core=rec[1])
     return out
 
 
@@ -280,46 +319,27 @@ def batched_mpint_leg(steps, warmup, vectors=4096, max_len=65536, seed=4):
     """4096 mpint vectors (32 moduli) of log-uniform length in [1, 65536], random signs, |x| < 2^480,
     op add/sub/mul drawn per vector: one persistent launch (autotuned, hipGraph) against one
     per-call launch per vector.  Every vector's result is compared between the two forms."""
-    import numpy as np
     import torch
 
     import grns_amd as G
-    from grns_amd.workload import Synth
+    from grns_amd.workload import MPINT_OPS, Synth, mpint_batch, mpint_view
 
     mod = G.largest_primes_below(N_MODULI)
     n = len(mod)
     ctx = G.Context(mod, 0)
-    syn = Synth(mod)
-    rng = np.random.default_rng(seed)
-    lens = np.floor(np.exp(rng.uniform(0, np.log(max_len), size=vectors))).astype(np.int64).clip(1, max_len)
-    ops = rng.integers(0, 3, size=vectors)
-    total = int(lens.sum())
-    offs = np.concatenate([[0], np.cumsum(lens)[:-1]])
-
-    def operand(stream):
-        bits = torch.full((total,), 480, dtype=torch.int32, device="cuda")
-        d = syn.bits(bits, seed, stream, top_set=False)
-        s = syn.uniform_u32(total, 0, 1, seed, stream + 100)
-        ev = ctx.eval(d)  # interval records have the mp_int_t eval[2] layout
-        return d, s, ev.view(total, 2, 2)
-
-    X, Y = operand(10), operand(20)
+    W = mpint_batch(ctx, Synth(mod), vectors=vectors, max_len=max_len, seed=seed)
+    X, Y, lens, offs, ops, total = W["X"], W["Y"], W["lens"], W["offs"], W["ops"], W["total"]
     Zb = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
     Zp = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
-    names = ["mpint_add", "mpint_sub", "mpint_mul"]
-
-    def view(T, o, ln):
-        return (T[0][:, o:o + ln], T[1][o:o + ln], T[2][o:o + ln])
-
-    tasks = [G.make_task(names[op], int(ln), mx=view(X, o, ln), my=view(Y, o, ln), mz=view(Zb, o, ln))
-             for op, o, ln in zip(ops, offs, lens)]
+    tasks = [G.make_task(MPINT_OPS[op], int(ln), mx=mpint_view(X, o, ln), my=mpint_view(Y, o, ln),
+                         mz=mpint_view(Zb, o, ln)) for op, o, ln in zip(ops, offs, lens)]
     batch = G.Batch(ctx, tasks)
     cfg, tuned_ms = batch.tune(reps=2)
     batch.set_config(use_graph=True)
 
     def plain():
         for op, o, ln in zip(ops, offs, lens):
-            ctx.mpint(names[op][6:], view(X, o, ln), view(Y, o, ln), view(Zp, o, ln), int(ln))
+            ctx.mpint(MPINT_OPS[op][6:], mpint_view(X, o, ln), mpint_view(Y, o, ln), mpint_view(Zp, o, ln), int(ln))
 
     plain()
     batch.launch()
@@ -341,92 +361,6 @@ def batched_mpint_leg(steps, warmup, vectors=4096, max_len=65536, seed=4):
     return out
 
 
-def serving_requests(n_req, seed, lo=10, hi=22):
-    """The serving workload's requests: each a dict with its context, batch tasks, a per-call
-    closure, served and per-call outputs, and its inputs (kept alive until the run ends)."""
-    import numpy as np
-    import torch
-
-    import grns_amd as G
-    from grns_amd.workload import Synth
-
-    rng = np.random.default_rng(seed)
-    sets = {k: G.largest_primes_below(k) for k in (8, 32, 64)}
-    ctx = {k: G.Context(m, 0) for k, m in sets.items()}
-    syn = {k: Synth(m) for k, m in sets.items()}
-    kinds = rng.choice(["evalcmp", "max", "div", "scale", "mpint"], size=n_req, p=[0.3, 0.2, 0.2, 0.1, 0.2])
-    sizes = np.floor(np.exp(rng.uniform(np.log(2 ** lo), np.log(2 ** hi), size=n_req))).astype(np.int64)
-    which = rng.choice([8, 32, 64], size=n_req)
-    reqs = []
-    elements = 0
-    for r, (kind, cnt, k) in enumerate(zip(kinds, sizes, which)):
-        cnt = int(cnt)
-        c, s, n = ctx[k], syn[k], int(k)
-        s_seed = seed * 1000 + r
-        if kind == "evalcmp":
-            x, y, _ = s.eval_cmp(cnt, s_seed)
-            outs = lambda: (torch.empty((cnt, 4), dtype=torch.float64, device="cuda"),
-                            torch.empty(cnt, dtype=torch.int32, device="cuda"))
-            o_s, o_p = outs(), outs()
-            tasks = [G.make_task("eval", cnt, a=x, out0=o_s[0]), G.make_task("cmp", cnt, a=x, b=y, out0=o_s[1])]
-
-            def plain(c=c, x=x, y=y, o=o_p):
-                c.eval(x, out=o[0])
-                c.cmp(x, y, out=o[1])
-            keep = (x, y)
-        elif kind == "max":
-            x, _, _ = s.max_planted(cnt, s_seed)
-            o_s = (torch.empty(n, dtype=torch.int32, device="cuda"), torch.empty(1, dtype=torch.int64, device="cuda"))
-            o_p = (torch.empty(n, dtype=torch.int32, device="cuda"), torch.empty(1, dtype=torch.int64, device="cuda"))
-            tasks = [G.make_task("max", cnt, a=x, out0=o_s[0], out1=o_s[1])]
-            ws = torch.empty(c.max_workspace_size(cnt), dtype=torch.uint8, device="cuda")
-
-            def plain(c=c, x=x, o=o_p, ws=ws):
-                c.max(x, workspace=ws, out=o[0], out_idx=o[1])
-            keep = (x,)
-        elif kind == "div":
-            L = G.product(sets[k]).bit_length() - 1
-            x, d, _, _ = s.div(cnt, seed=s_seed, max_bits=L - 1, max_shift=L)
-            o_s = (torch.empty_like(x), torch.empty_like(x))
-            o_p = (torch.empty_like(x), torch.empty_like(x))
-            tasks = [G.make_task("div_fast", cnt, a=x, b=d, out0=o_s[0], out1=o_s[1])]
-
-            def plain(c=c, x=x, d=d, o=o_p):
-                c.div(x, d, fast=True, q=o[0], r=o[1])
-            keep = (x, d)
-        elif kind == "scale":
-            x = s.uniform(cnt, s_seed, 8)
-            D = s.uniform_u32(cnt, 1, 1900, s_seed, 9)
-            o_s, o_p = (torch.empty_like(x),), (torch.empty_like(x),)
-            tasks = [G.make_task("scale2pow", cnt, a=x, out0=o_s[0], D=D)]
-
-            def plain(c=c, x=x, D=D, o=o_p):
-                c.scale2pow(x, D, y=o[0])
-            keep = (x, D)
-        else:
-            L = G.product(sets[k]).bit_length() - 1
-            bits = torch.full((cnt,), min(480, (L - 4) // 2), dtype=torch.int32, device="cuda")
-
-            def operand(st):
-                dg = s.bits(bits, s_seed, st, top_set=False)
-                sg = s.uniform_u32(cnt, 0, 1, s_seed, st + 100)
-                return dg, sg, c.eval(dg).view(cnt, 2, 2)
-            X, Y = operand(10), operand(20)
-            o_s = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
-            o_p = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
-            tasks = [G.make_task("mpint_mul", cnt, mx=X, my=Y, mz=o_s)]
-
-            def plain(c=c, X=X, Y=Y, o=o_p, cnt=cnt):
-                c.mpint("mul", X, Y, o, cnt)
-            keep = (X, Y)
-        elements += cnt
-        reqs.append({"ctx": c, "tasks": G.Server.task_array(tasks), "plain": plain, "out_s": o_s, "out_p": o_p,
-                     "keep": keep,
-                     "kind": kind, "n": n, "count": cnt})
-    torch.cuda.synchronize()
-    return reqs, elements, ctx
-
-
 def serving_leg(n_req=120, seed=7, budget=8 << 30, lo=10, hi=22, max_batch=16):
     """Serving stream: requests with Poisson arrivals over three resident moduli
     sets (8, 32, 64 largest primes below 2^31), op mix 30% eval+cmp, 20% arg-max, 20% div_fast,
@@ -440,7 +374,9 @@ def serving_leg(n_req=120, seed=7, budget=8 << 30, lo=10, hi=22, max_batch=16):
 
     import grns_amd as G
 
-    reqs, elements, _ctx = serving_requests(n_req, seed, lo, hi)
+    from grns_amd.workload import serving_requests
+    reqs, _ctx = serving_requests(n_req, seed, lo, hi)
+    elements = sum(q["count"] for q in reqs)
     rng = np.random.default_rng(seed + 1)
 
     # plain per-call path: one request after another on the current stream
@@ -641,26 +577,12 @@ def main():
     # ------------------------------------------------------------------ CPU baseline
     cpu = None
     if not args.no_cpu:
-        from oracle.oracle import Oracle
-        orc = Oracle(mod)
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-        sample = min(N, 1 << 22)
-        xh = syn.max_planted(sample, seed=2)[0].cpu().numpy()
-        # repeated passes over the same array until about 12 s of CPU work have been timed
-        passes, t0 = 0, time.perf_counter()
-        while True:
-            orc.max_eval_array(xh, nthreads=threads)
-            passes += 1
-            dt = time.perf_counter() - t0
-            if dt >= 12.0 or passes >= 400:
-                break
-        cpu = {"value": round(passes * sample / dt), "unit": "RNS ops/s", "cores": threads, "kind": "port",
-               "sample": f"rns_max pipeline (evaluate + rns_max_cmp reduction, CPU path) over 2^"
-                         f"{sample.bit_length() - 1} numbers of the same workload, {passes} passes, "
-                         f"{threads} OpenMP threads, {dt:.2f}s"}
-        log(f"[bench] cpu baseline {cpu['value']} ops/s on {threads} threads")
+        threads = cpu_threads()
+        cpu = cpu_headline(mod, syn, N)
+        log(f"[bench] cpu baseline {cpu['value']} ops/s on {threads} threads, {cpu['one_core']['value']} on one")
         extras["cpu_per_op"] = cpu_per_op(threads)
-        log(f"[bench] cpu per op: " + ", ".join(f"{k} {v['value']:.3g}/s" for k, v in extras["cpu_per_op"].items()))
+        log("[bench] cpu per op: " + ", ".join(f"{k} {v['value']:.3g}/s ({v['one_core']['value']:.3g} on one)"
+                                               for k, v in extras["cpu_per_op"].items()))
 
     value = N / (ms_max * 1e-3)
     rl = roofline(max_bytes, kms_max, "k_max_partial<32, 1>")

@@ -182,3 +182,133 @@ class Synth:
         d = self.bits(nb, seed, 6, top_set=True)
         D = self.uniform_u32(count, 1, max_shift, seed, 7)
         return x, d, D, nb
+
+
+# ---------------------------------------------------------------------- multi-request workloads
+def mpint_batch(ctx, syn, vectors=4096, max_len=65536, seed=4, bits=480):
+    """The batched mpint workload: `vectors` vectors with lengths floor(e^U[0, ln max_len]) (log-uniform
+    in [1, max_len]) laid end to end, an operation (0 add, 1 sub, 2 mul) drawn per vector, operands
+    X and Y of uniform `bits`-bit magnitudes with random signs and their evaluations (ctx.eval) as the
+    eval fields.  Returns a dict: X, Y ((digits (n, total), sign (total,), eval (total, 2, 2))), lens,
+    offs, ops (numpy arrays) and total."""
+    import torch
+    rng = np.random.default_rng(seed)
+    lens = np.floor(np.exp(rng.uniform(0, np.log(max_len), size=vectors))).astype(np.int64).clip(1, max_len)
+    ops = rng.integers(0, 3, size=vectors)
+    total = int(lens.sum())
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+
+    def operand(stream):
+        b = torch.full((total,), bits, dtype=torch.int32, device=syn.device)
+        d = syn.bits(b, seed, stream, top_set=False)
+        s = syn.uniform_u32(total, 0, 1, seed, stream + 100)
+        ev = ctx.eval(d)  # interval records have the mp_int_t eval[2] layout
+        return d, s, ev.view(total, 2, 2)
+
+    return {"X": operand(10), "Y": operand(20), "lens": lens, "offs": offs, "ops": ops, "total": total}
+
+
+MPINT_OPS = ["mpint_add", "mpint_sub", "mpint_mul"]
+
+
+def mpint_view(T, off, length):
+    """The (digits, sign, eval) slices of one vector of an mpint operand triple."""
+    return T[0][:, off:off + length], T[1][off:off + length], T[2][off:off + length]
+
+
+SERVING_MIX = (("evalcmp", 0.3), ("max", 0.2), ("div", 0.2), ("scale", 0.1), ("mpint", 0.2))
+
+
+def serving_requests(n_req, seed, lo=10, hi=22, sets=(8, 32, 64)):
+    """The serving workload's requests: a mix of 30% eval+cmp, 20% arg-max, 20% div_fast, 10%
+    scale2pow and 20% mpint mul, sizes floor(e^U[ln 2^lo, ln 2^hi]), each on one of the resident
+    moduli sets of 8, 32 and 64 largest primes below 2^31, operands drawn like the single-operation
+    workloads (edge values in eval+cmp, a planted maximum with near-ties in arg-max, divisors of
+    uniform bit length, exponents in [1, 1900], mpint magnitudes of up to 480 bits within the range).
+
+    Each request is a dict: ctx, tasks (grns_batch_task array), plain (a closure running it through
+    the per-call entry points into out_p), out_s (the served outputs), out_p, keep (its inputs, alive
+    until the run ends), kind, n, count, and truth (for arg-max: planted index and value; for the
+    others: the input tensors the checker samples)."""
+    import torch
+
+    import grns_amd as G
+
+    rng = np.random.default_rng(seed)
+    setmod = {k: G.largest_primes_below(k) for k in sets}
+    ctx = {k: G.Context(m, 0) for k, m in setmod.items()}
+    syn = {k: Synth(m) for k, m in setmod.items()}
+    names = [k for k, _ in SERVING_MIX]
+    kinds = rng.choice(names, size=n_req, p=[p for _, p in SERVING_MIX])
+    sizes = np.floor(np.exp(rng.uniform(np.log(2 ** lo), np.log(2 ** hi), size=n_req))).astype(np.int64)
+    which = rng.choice(list(sets), size=n_req)
+    reqs = []
+    for r, (kind, cnt, k) in enumerate(zip(kinds, sizes, which)):
+        cnt = int(cnt)
+        c, s, n = ctx[k], syn[k], int(k)
+        s_seed = seed * 1000 + r
+        truth = {}
+        if kind == "evalcmp":
+            x, y, _ = s.eval_cmp(cnt, s_seed)
+
+            def outs():
+                return (torch.empty((cnt, 4), dtype=torch.float64, device="cuda"),
+                        torch.empty(cnt, dtype=torch.int32, device="cuda"))
+            o_s, o_p = outs(), outs()
+            tasks = [G.make_task("eval", cnt, a=x, out0=o_s[0]), G.make_task("cmp", cnt, a=x, b=y, out0=o_s[1])]
+
+            def plain(c=c, x=x, y=y, o=o_p):
+                c.eval(x, out=o[0])
+                c.cmp(x, y, out=o[1])
+            keep = (x, y)
+        elif kind == "max":
+            x, p, V = s.max_planted(cnt, s_seed)
+            truth = {"index": p, "value": V}
+            o_s = (torch.empty(n, dtype=torch.int32, device="cuda"), torch.empty(1, dtype=torch.int64, device="cuda"))
+            o_p = (torch.empty(n, dtype=torch.int32, device="cuda"), torch.empty(1, dtype=torch.int64, device="cuda"))
+            tasks = [G.make_task("max", cnt, a=x, out0=o_s[0], out1=o_s[1])]
+            ws = torch.empty(c.max_workspace_size(cnt), dtype=torch.uint8, device="cuda")
+
+            def plain(c=c, x=x, o=o_p, ws=ws):
+                c.max(x, workspace=ws, out=o[0], out_idx=o[1])
+            keep = (x, ws)
+        elif kind == "div":
+            L = product(setmod[k]).bit_length() - 1
+            x, d, _, _ = s.div(cnt, seed=s_seed, max_bits=L - 1, max_shift=L)
+            o_s = (torch.empty_like(x), torch.empty_like(x))
+            o_p = (torch.empty_like(x), torch.empty_like(x))
+            tasks = [G.make_task("div_fast", cnt, a=x, b=d, out0=o_s[0], out1=o_s[1])]
+
+            def plain(c=c, x=x, d=d, o=o_p):
+                c.div(x, d, fast=True, q=o[0], r=o[1])
+            keep = (x, d)
+        elif kind == "scale":
+            x = s.uniform(cnt, s_seed, 8)
+            D = s.uniform_u32(cnt, 1, 1900, s_seed, 9)
+            o_s, o_p = (torch.empty_like(x),), (torch.empty_like(x),)
+            tasks = [G.make_task("scale2pow", cnt, a=x, out0=o_s[0], D=D)]
+
+            def plain(c=c, x=x, D=D, o=o_p):
+                c.scale2pow(x, D, y=o[0])
+            keep = (x, D)
+        else:
+            L = product(setmod[k]).bit_length() - 1
+            bits = torch.full((cnt,), min(480, (L - 4) // 2), dtype=torch.int32, device="cuda")
+
+            def operand(st, c=c, s=s, bits=bits, s_seed=s_seed, cnt=cnt):
+                dg = s.bits(bits, s_seed, st, top_set=False)
+                sg = s.uniform_u32(cnt, 0, 1, s_seed, st + 100)
+                return dg, sg, c.eval(dg).view(cnt, 2, 2)
+            X, Y = operand(10), operand(20)
+            o_s = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
+            o_p = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
+            tasks = [G.make_task("mpint_mul", cnt, mx=X, my=Y, mz=o_s)]
+
+            def plain(c=c, X=X, Y=Y, o=o_p, cnt=cnt):
+                c.mpint("mul", X, Y, o, cnt)
+            keep = (X, Y)
+        reqs.append({"ctx": c, "tasks": G.Server.task_array(tasks), "task_list": tasks, "plain": plain,
+                     "out_s": o_s, "out_p": o_p, "keep": keep, "kind": str(kind), "n": n, "count": cnt,
+                     "truth": truth, "moduli": setmod[k]})
+    torch.cuda.synchronize()
+    return reqs, ctx

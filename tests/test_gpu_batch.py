@@ -23,6 +23,7 @@ def _d(a):
 
 
 def _mp_operands(o, mod, cnt, seed, bits):
+    """Device (digits, sign, eval) of random signed magnitudes, and the same as numpy arrays."""
     rnd = random.Random(seed)
     mags = [rnd.randrange(1 << rnd.randrange(1, bits)) for _ in range(cnt)]
     signs = np.array([rnd.randrange(2) if m else 0 for m in mags], np.int32)
@@ -30,7 +31,7 @@ def _mp_operands(o, mod, cnt, seed, bits):
     ev = o.eval_array(d)
     e = np.zeros((cnt, 2), dtype=ev["low"].dtype)
     e[:, 0], e[:, 1] = ev["low"], ev["upp"]
-    return _d(d), _d(signs), _d(e.view(np.float64).reshape(cnt, 2, 2))
+    return (_d(d), _d(signs), _d(e.view(np.float64).reshape(cnt, 2, 2))), (d, signs, e)
 
 
 def _build(n, seed):
@@ -41,32 +42,38 @@ def _build(n, seed):
     ctx = G.Context(mod, 0)
     o = Oracle(mod)
     rnd = random.Random(seed)
-    jobs = []  # (kind, inputs, batch outputs, plain outputs)
+    jobs = []  # (kind, inputs, batch outputs, the oracle's output bytes)
     for k in range(12):
         cnt = rnd.choice([1, 3, 63, 64, 65, 255, 700, 2000])
         kind = ["eval", "eval_fast", "cmp", "mpint_add", "mpint_sub", "mpint_mul"][k % 6]
         if kind.startswith("eval"):
-            x = _d(to_res(eval_inputs(mod, cnt, seed + k), mod))
-            jobs.append((kind, (x,), torch.zeros((cnt, 4), dtype=torch.float64, device="cuda"), None))
+            xn = to_res(eval_inputs(mod, cnt, seed + k), mod)
+            want = o.eval_array(xn, fast=(kind == "eval_fast")).tobytes()
+            jobs.append((kind, (_d(xn),), torch.zeros((cnt, 4), dtype=torch.float64, device="cuda"), want))
         elif kind == "cmp":
             xs, ys = cmp_pairs(mod, cnt, seed + k)
-            jobs.append((kind, (_d(to_res(xs, mod)), _d(to_res(ys, mod))),
-                         torch.zeros(cnt, dtype=torch.int32, device="cuda"), None))
+            a, b = to_res(xs, mod), to_res(ys, mod)
+            jobs.append((kind, (_d(a), _d(b)), torch.zeros(cnt, dtype=torch.int32, device="cuda"),
+                         o.cmp_array(a, b).astype(np.int32).tobytes()))
         else:
             half = (o.log2M - 2) // 2
-            X = _mp_operands(o, mod, cnt, seed + 3 * k, half)
-            Y = _mp_operands(o, mod, cnt, seed + 3 * k + 1, half)
+            X, Xn = _mp_operands(o, mod, cnt, seed + 3 * k, half)
+            Y, Yn = _mp_operands(o, mod, cnt, seed + 3 * k + 1, half)
             Z = (torch.zeros_like(X[0]), torch.zeros_like(X[1]), torch.zeros_like(X[2]))
             of = torch.zeros(cnt, dtype=torch.uint8, device="cuda")
-            jobs.append((kind, (X, Y), (Z, of), None))
+            rd, rs, re, ro = o.mpint_array(kind.split("_")[1], *Xn, *Yn)
+            jobs.append((kind, (X, Y), (Z, of), b"".join(a.tobytes() for a in (rd, rs, re, ro))))
     dcnt = {8: 40, 32: 12, 64: 6}[n]
     xs, ds = div_inputs(mod, dcnt, seed)
-    xd, dd = _d(to_res(xs, mod)), _d(to_res(ds, mod))
-    jobs.append(("div_fast", (xd, dd), (torch.zeros_like(xd), torch.zeros_like(xd)), None))
+    xn, dn = to_res(xs, mod), to_res(ds, mod)
+    xd, dd = _d(xn), _d(dn)
+    q, r = o.div_array(xn, dn, fast=True)
+    jobs.append(("div_fast", (xd, dd), (torch.zeros_like(xd), torch.zeros_like(xd)), q.tobytes() + r.tobytes()))
     vals = eval_inputs(mod, 300, seed + 99)
-    xsc = _d(to_res(vals, mod))
-    D = _d(np.array([rnd.randrange(0, o.log2M + 3) for _ in vals], np.int32))
-    jobs.append(("scale2pow", (xsc, D), torch.zeros_like(xsc), None))
+    xn = to_res(vals, mod)
+    Dn = np.array([rnd.randrange(0, o.log2M + 3) for _ in vals], np.int32)
+    jobs.append(("scale2pow", (_d(xn), _d(Dn)), torch.zeros_like(_d(xn)),
+                 o.scale2pow_array(xn, Dn.astype(np.uint32)).tobytes()))
     return ctx, o, jobs
 
 
@@ -144,7 +151,10 @@ def test_batch_matches_plain(dev, n, tpn, tile, bpc):
     b.set_config(tile=tile, blocks_per_cu=bpc, threads_per_number=tpn, use_graph=False)
     b.launch()
     torch.cuda.synchronize()
-    assert _got(jobs) == ref
+    got = _got(jobs)
+    assert got == ref
+    for (kind, _, _, want), g in zip(jobs, got):
+        assert g == want, kind  # and the oracle's results
     assert ctx.status() == 0
     b.close()
 

@@ -34,7 +34,9 @@ class Req:
         cnt = rnd.choice([1, 17, 256, 1000, 3000])
         if kind == "evalcmp":
             xs, ys = cmp_pairs(mod, cnt, seed)
-            x, y = _d(to_res(xs, mod)), _d(to_res(ys, mod))
+            xn, yn = to_res(xs, mod), to_res(ys, mod)
+            x, y = _d(xn), _d(yn)
+            self.oracle = [o.eval_array(xn).tobytes(), o.cmp_array(xn, yn).astype(np.int32).tobytes()]
             self.keep += [x, y]
             ev = torch.zeros((cnt, 4), dtype=torch.float64, device="cuda")
             sg = torch.zeros(cnt, dtype=torch.int32, device="cuda")
@@ -44,6 +46,9 @@ class Req:
         elif kind == "max":
             vals = eval_inputs(mod, cnt, seed)
             x = _d(to_res(vals, mod))
+            best = max(range(cnt), key=lambda j: (vals[j], -j))  # the lowest index among equal maxima
+            self.oracle = [np.array([vals[best] % m for m in mod], np.int32).tobytes(),
+                           np.array([best], np.int64).tobytes()]
             self.keep.append(x)
             r = torch.zeros(len(mod), dtype=torch.int32, device="cuda")
             i = torch.zeros(1, dtype=torch.int64, device="cuda")
@@ -53,7 +58,9 @@ class Req:
         elif kind == "div":
             k = max(1, cnt // {4: 1, 8: 20, 32: 100, 64: 300}[len(mod)])
             xs, ds = div_inputs(mod, k, seed)
-            x, d = _d(to_res(xs, mod)), _d(to_res(ds, mod))
+            xn, dn = to_res(xs, mod), to_res(ds, mod)
+            x, d = _d(xn), _d(dn)
+            self.oracle = [a.tobytes() for a in o.div_array(xn, dn, fast=True)]
             self.keep += [x, d]
             q, r = torch.zeros_like(x), torch.zeros_like(x)
             self.tasks = [G.make_task("div_fast", x.shape[1], a=x, b=d, out0=q, out1=r)]
@@ -61,8 +68,10 @@ class Req:
             self.expect = list(ctx.div(x, d, fast=True))
         elif kind == "scale":
             vals = eval_inputs(mod, cnt, seed)
-            x = _d(to_res(vals, mod))
-            D = _d(np.array([rnd.randrange(0, 80) for _ in vals], np.int32))
+            xn = to_res(vals, mod)
+            Dn = np.array([rnd.randrange(0, 80) for _ in vals], np.int32)
+            x, D = _d(xn), _d(Dn)
+            self.oracle = [o.scale2pow_array(xn, Dn.astype(np.uint32)).tobytes()]
             self.keep += [x, D]
             y = torch.zeros_like(x)
             self.tasks = [G.make_task("scale2pow", cnt, a=x, out0=y, D=D)]
@@ -76,6 +85,8 @@ class Req:
             e = np.zeros((cnt, 2), dtype=ev["low"].dtype)
             e[:, 0], e[:, 1] = ev["low"], ev["upp"]
             X = (_d(dg), _d(sg), _d(e.view(np.float64).reshape(cnt, 2, 2)))
+            rd, rs, re, _ = o.mpint_array("mul", dg, sg, e, dg, sg, e)
+            self.oracle = [rd.tobytes(), rs.tobytes(), re.tobytes()]
             self.keep += list(X)
             Z = (torch.zeros_like(X[0]), torch.zeros_like(X[1]), torch.zeros_like(X[2]))
             self.tasks = [G.make_task("mpint_mul", cnt, mx=X, my=X, mz=Z)]
@@ -86,8 +97,11 @@ class Req:
         torch.cuda.synchronize()
 
     def check(self):
-        for a, b in zip(self.outs, self.expect):
-            assert a.cpu().numpy().tobytes() == b.cpu().numpy().tobytes(), self.kind
+        """Served outputs == the request run alone through the per-call path == the oracle."""
+        for a, b, w in zip(self.outs, self.expect, self.oracle):
+            got = a.cpu().numpy().tobytes()
+            assert got == b.cpu().numpy().tobytes(), self.kind
+            assert got == w, self.kind + " (oracle)"
 
 
 def _requests(n_req, seed):
