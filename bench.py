@@ -353,7 +353,7 @@ def batched_mpint_leg(steps, warmup, vectors=4096, max_len=65536, seed=4):
     ms_p = timed(plain, max(2, steps // 4), 1)
     bytes_per = 3 * (4 * n + 36)
     out = {"workload": f"{vectors} mpint vectors n={n}, log-uniform lengths [1,{max_len}], {total} elements, "
-                       "add/sub/mul per vector",
+                       "add/sub/mul per vector, digit rows 256-byte aligned",
            "elements": total, "batched_ms": round(ms_b, 4), "per_call_ms": round(ms_p, 4),
            "speedup": round(ms_p / ms_b, 2), "elem_ops_per_s": round(total / (ms_b * 1e-3)),
            "roofline": roofline(bytes_per * total, ms_b, "k_batch_thread"), "tuned": cfg, "bit_exact_vs_per_call": same}

@@ -72,6 +72,7 @@ EXPORTS = [
     "grns_version", "grns_ctx_create", "grns_ctx_destroy", "grns_ctx_n", "grns_ctx_log2m", "grns_ctx_constants",
     "grns_ctx_tables", "grns_ctx_status", "grns_eval", "grns_cmp", "grns_div", "grns_scale2pow",
     "grns_max_workspace_size", "grns_max", "grns_max_noalloc", "grns_max_mrc", "grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul", "grns_mpint_div",
+    "grns_mpint_aos", "grns_mpint_aos_bytes",
     "grns_batch_create", "grns_batch_set_config", "grns_batch_get_config", "grns_batch_tune", "grns_batch_launch",
     "grns_batch_destroy", "grns_serve_create", "grns_serve_submit", "grns_serve_poll", "grns_serve_wait",
     "grns_serve_drain", "grns_serve_stats_get", "grns_serve_destroy", "grns_ctx_words", "grns_from_binary",
@@ -83,7 +84,7 @@ DEBUG_EXPORTS = [
     "grns_debug_mulmod_recip_d",
     "grns_debug_mulmod_recip", "grns_debug_addmod_recip", "grns_debug_bn_err", "grns_debug_fast_scalbn",
     "grns_debug_filter_tau", "grns_debug_filter_err", "grns_debug_stage_timing", "grns_debug_stage_read",
-    "grns_debug_reference_loops",
+    "grns_debug_reference_loops", "grns_debug_stream_tiles", "grns_debug_grid_cap",
 ]
 
 
@@ -117,6 +118,9 @@ def lib():
     L.grns_max_workspace_size.argtypes = [P, SZ]
     L.grns_max.argtypes = [P, P, LL, LL, SZ, P, P, P, SZ, P]
     L.grns_max_noalloc.argtypes = [P, P, LL, LL, SZ, P, P, P]
+    L.grns_mpint_aos_bytes.restype = SZ
+    L.grns_mpint_aos_bytes.argtypes = [P]
+    L.grns_mpint_aos.argtypes = [P, I, P, P, P, SZ, SZ, P, P]
     L.grns_max_mrc.argtypes = [P, P, LL, LL, SZ, P, P, P, SZ, P]
     for nm in ("grns_mpint_add", "grns_mpint_sub", "grns_mpint_mul", "grns_mpint_div"):
         getattr(L, nm).argtypes = [P, C.POINTER(MpintArray), C.POINTER(MpintArray), C.POINTER(MpintArray), SZ, P, P]
@@ -151,6 +155,8 @@ def lib():
     L.grns_debug_filter_err.argtypes = [P]
     L.grns_debug_stage_timing.argtypes = [P, I]
     L.grns_debug_reference_loops.argtypes = [P, I]
+    L.grns_debug_stream_tiles.argtypes = [P, I]
+    L.grns_debug_grid_cap.argtypes = [P, I]
     L.grns_debug_stage_read.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     _lib = L
     return L
@@ -365,6 +371,14 @@ class Context:
         xa, ya, za = mpint_array(*x), mpint_array(*y), mpint_array(*z)
         _check(fn(self._h, C.byref(xa), C.byref(ya), C.byref(za), count,
                   _tptr(overflow) if overflow is not None else None, _stream_ptr(stream)), f"grns_mpint_{op}")
+
+    def stream_tiles(self, on=True):
+        """LDS-staged tiles (on) or direct loads (off, the default) in eval (test hook)."""
+        _check(lib().grns_debug_stream_tiles(self._h, int(on)), "grns_debug_stream_tiles")
+
+    def grid_cap(self, blocks_per_cu=0):
+        """Cap the streaming kernels' grids at blocks_per_cu blocks per CU, 0 for none (test hook)."""
+        _check(lib().grns_debug_grid_cap(self._h, int(blocks_per_cu)), "grns_debug_grid_cap")
 
     def reference_loops(self, on=True):
         """Division and scaling through the reference's loops for every number (test hook)."""

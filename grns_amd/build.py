@@ -7,6 +7,7 @@ reference's host code; the few fused multiply-adds the kernels need are explicit
 from __future__ import annotations
 
 import os
+import shutil
 import subprocess
 import sys
 
@@ -45,15 +46,28 @@ def _run(cmds, verbose):
         raise subprocess.CalledProcessError(1, failed[0])
 
 
+def _snapshot(obj_dir: str) -> str:
+    """Copy of the sources (csrc/ and include/) that the compilers read: hipcc reads a source and
+    its headers once for the device code and again, minutes later, for the host code, so a file
+    edited in between would give a library whose host side launches kernels its device code lacks."""
+    src = os.path.join(obj_dir, "src")
+    if os.path.isdir(src):
+        shutil.rmtree(src)
+    shutil.copytree(CSRC, os.path.join(src, "grns_amd", "csrc"))
+    shutil.copytree(os.path.join(os.path.dirname(HERE), "include"), os.path.join(src, "include"))
+    return os.path.join(src, "grns_amd", "csrc")
+
+
 def build(verbose: bool = False) -> str:
-    """Compile libgrns_hip.so (the hot path) and libgrns_workload.so (synthetic inputs): every
-    source to an object in parallel, then one link per library."""
+    """Compile libgrns_hip.so (the hot path) and libgrns_workload.so (synthetic inputs) from one
+    snapshot of the sources: every source to an object in parallel, then one link per library."""
     obj_dir = os.path.join(HERE, "_build")
     os.makedirs(obj_dir, exist_ok=True)
+    csrc = _snapshot(obj_dir)
     compile_flags = [f for f in FLAGS if f != "-shared"]
     units = SOURCES + ["grns_workload.hip"]
     objs = {u: os.path.join(obj_dir, u.replace(".hip", ".o")) for u in units}
-    _run([[HIPCC, *compile_flags, "-c", "-o", objs[u], os.path.join(CSRC, u)] for u in units], verbose)
+    _run([[HIPCC, *compile_flags, "-c", "-o", objs[u], os.path.join(csrc, u)] for u in units], verbose)
     links = [(LIB, [objs[u] for u in SOURCES]), (WORKLOAD_LIB, [objs["grns_workload.hip"]])]
     _run([[HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp", *o] for out, o in links], verbose)
     for out, _ in links:

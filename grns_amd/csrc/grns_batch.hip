@@ -79,9 +79,13 @@ GRNS_D bool next_tile(const BatchTask *__restrict__ tasks, int ntasks, long long
     return t < total;
 }
 
-// Op classes of a thread-list launch: the kernel holds only the code of the classes its tasks use,
-// so a list of mpint tasks runs with the registers (and occupancy) of the mpint code alone.
-enum : int { OPS_EVALCMP = 1, OPS_MPINT = 2 };
+// Operation class of a thread-mapped task list: one kernel instantiation per class, so that each
+// carries only its class's code and runs with its registers and occupancy (as the per-call kernels
+// do): evaluation (and its fast form), comparison, and the mpint add / sub / mul, whose codes need
+// about the same registers (104-114 VGPRs at n = 32) and so share one kernel and one tile space.
+enum : int { OPS_EVAL = 1, OPS_CMP = 2, OPS_MPINT = 4 };
+constexpr int OP_CLASSES = 3;
+constexpr int BATCH_COUNTERS = 4;  // tile counters: one per thread list (3), one for the group list
 
 template <int N, int OPS>
 __global__ __launch_bounds__(BLOCK) void k_batch_thread(const DevCtx *__restrict__ c, const BatchTask *__restrict__ tasks,
@@ -96,33 +100,33 @@ __global__ __launch_bounds__(BLOCK) void k_batch_thread(const DevCtx *__restrict
         switch (T.op) {
             case GRNS_OP_EVAL:
             case GRNS_OP_EVAL_FAST:
-                if constexpr ((OPS & OPS_EVALCMP) != 0) {
+                if constexpr ((OPS & OPS_EVAL) != 0) {
                     interval_t *o = reinterpret_cast<interval_t *>(T.o0);
                     for (long long base = e0; base < e1; base += BLOCK) {
                         if (T.op == GRNS_OP_EVAL)
-                            eval_step<N, false>(c, T.a, T.asm_, T.asn, base + threadIdx.x, e1, o, status);
-                        else eval_step<N, true>(c, T.a, T.asm_, T.asn, base + threadIdx.x, e1, o, status);
+                            eval_step<N, false, LD_MM>(c, T.a, T.asm_, 1, base + threadIdx.x, e1, o, status);
+                        else eval_step<N, true, LD_MM>(c, T.a, T.asm_, 1, base + threadIdx.x, e1, o, status);
                     }
                 }
                 break;
             case GRNS_OP_CMP:
-                if constexpr ((OPS & OPS_EVALCMP) != 0) {
+                if constexpr ((OPS & OPS_CMP) != 0) {
                     for (long long base = e0; base < e1; base += BLOCK)
-                        cmp_step<N>(c, T.a, T.asm_, T.asn, T.b, T.bsm, T.bsn, base + threadIdx.x, e1,
-                                    reinterpret_cast<int *>(T.o0), status);
+                        cmp_step<N, LD_MM>(c, T.a, T.asm_, 1, T.b, T.bsm, 1, base + threadIdx.x, e1,
+                                           reinterpret_cast<int *>(T.o0), status);
                 }
                 break;
             case GRNS_OP_MPINT_ADD:
                 if constexpr ((OPS & OPS_MPINT) != 0)
-                    for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 0>(c, T.mp, j);
+                    for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 0, true>(c, T.mp, j);
                 break;
             case GRNS_OP_MPINT_SUB:
                 if constexpr ((OPS & OPS_MPINT) != 0)
-                    for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 1>(c, T.mp, j);
+                    for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 1, true>(c, T.mp, j);
                 break;
             case GRNS_OP_MPINT_MUL:
                 if constexpr ((OPS & OPS_MPINT) != 0)
-                    for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 2>(c, T.mp, j);
+                    for (long long j = e0 + threadIdx.x; j < e1; j += BLOCK) mpint_step<N, 2, true>(c, T.mp, j);
                 break;
             default:
                 break;
@@ -218,7 +222,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_group(const DevCtx *__restrict_
 }
 
 __global__ void k_batch_reset(unsigned long long *counters) {
-    if (threadIdx.x < 2) counters[threadIdx.x] = 0;
+    if (threadIdx.x < BATCH_COUNTERS) counters[threadIdx.x] = 0;
 }
 
 }  // namespace grns
@@ -228,6 +232,15 @@ namespace {
 bool thread_kernel_available(int n) { return n == 4 || n == 8 || n == 16 || n == 32 || n == 64; }
 
 bool elementwise(int op) { return op >= GRNS_OP_EVAL && op <= GRNS_OP_MPINT_MUL; }
+
+// Thread lists address digit rows by 32-bit lane offsets from scalar row pointers: numbers stored
+// digit-major (number stride 1) and at most 2^29 of them.  Other layouts go to the group list.
+bool lane_layout(const grns_batch_task &u) {
+    if (u.count > ((size_t)1 << 29)) return false;
+    if (u.op == GRNS_OP_EVAL || u.op == GRNS_OP_EVAL_FAST) return u.a_num_stride == 1;
+    if (u.op == GRNS_OP_CMP) return u.a_num_stride == 1 && u.b_num_stride == 1;
+    return true;
+}
 
 BatchTask to_dev(const grns_batch_task &u) {
     BatchTask t;
@@ -245,79 +258,109 @@ BatchTask to_dev(const grns_batch_task &u) {
     return t;
 }
 
-// Tasks split by mapping (thread-per-number first, then group-per-number), with the tile prefix
-// of each list filled in.
+// Tasks split by mapping and operation: the thread-per-number lists first, one per operation, then
+// the group-per-number list; each list has its own tile prefix (tile0 counts from 0 within it).
+struct ThreadList {
+    int ops;          // OPS_* of the list's operation
+    int first;        // index of its first task in TaskPlan::all
+    int count;        // tasks
+    long long tiles;  // tiles of `tile` numbers
+};
+
 struct TaskPlan {
     std::vector<BatchTask> all;
+    std::vector<ThreadList> lists;
     int nthr = 0, ngrp = 0;
-    long long thr_tiles = 0, grp_tiles = 0;
+    long long grp_tiles = 0;
     long long grp_tile = 1;  // numbers per tile of the group list: a group does far more work per number
-    int thr_ops = 0;         // op classes (OPS_*) of the thread list
 };
+
+int op_class(int op) {
+    switch (op) {
+        case GRNS_OP_EVAL:
+        case GRNS_OP_EVAL_FAST: return OPS_EVAL;
+        case GRNS_OP_CMP: return OPS_CMP;
+        default: return OPS_MPINT;
+    }
+}
 
 void make_plan(const grns_batch_task *u, int n, int ctx_n, const grns_batch_config &cfg, TaskPlan &P) {
     P.all.clear();
-    std::vector<BatchTask> grp;
+    P.lists.clear();
+    std::vector<BatchTask> grp, by_op[OP_CLASSES];
     const bool thread_ok = thread_kernel_available(ctx_n) && cfg.threads_per_number == 1;
     const long long tile = cfg.tile;
     P.grp_tile = tile / 16 > 0 ? tile / 16 : 1;
-    P.thr_tiles = P.grp_tiles = 0;
-    P.thr_ops = 0;
+    P.grp_tiles = 0;
     for (int i = 0; i < n; i++) {
         if (u[i].count == 0 || u[i].op == GRNS_OP_MAX) continue;
         BatchTask t = to_dev(u[i]);
-        if (thread_ok && elementwise(u[i].op)) {
-            t.tile0 = P.thr_tiles;
-            P.thr_tiles += (t.count + tile - 1) / tile;
-            P.thr_ops |= (u[i].op == GRNS_OP_MPINT_ADD || u[i].op == GRNS_OP_MPINT_SUB || u[i].op == GRNS_OP_MPINT_MUL)
-                             ? OPS_MPINT
-                             : OPS_EVALCMP;
-            P.all.push_back(t);
+        if (thread_ok && elementwise(u[i].op) && lane_layout(u[i])) {
+            by_op[__builtin_ctz(op_class(u[i].op))].push_back(t);
         } else {
             t.tile0 = P.grp_tiles;
             P.grp_tiles += (t.count + P.grp_tile - 1) / P.grp_tile;
             grp.push_back(t);
         }
     }
+    for (int k = 0; k < OP_CLASSES; k++) {
+        if (by_op[k].empty()) continue;
+        ThreadList L{1 << k, (int)P.all.size(), (int)by_op[k].size(), 0};
+        for (BatchTask &t : by_op[k]) {
+            t.tile0 = L.tiles;
+            L.tiles += (t.count + tile - 1) / tile;
+            P.all.push_back(t);
+        }
+        P.lists.push_back(L);
+    }
     P.nthr = (int)P.all.size();
     P.ngrp = (int)grp.size();
     P.all.insert(P.all.end(), grp.begin(), grp.end());
 }
 
-// Counter reset plus one persistent launch per non-empty list; d_tasks must hold P.all.
+// Counter reset plus one persistent launch per non-empty list, in order on the stream; d_tasks
+// must hold P.all.
+template <int NN>
+void launch_thread_list(grns_ctx *c, const ThreadList &L, const BatchTask *d_tasks, unsigned long long *counter,
+                        int tile, dim3 grid, hipStream_t s) {
+    const dim3 blk(BLOCK);
+    const BatchTask *tl = d_tasks + L.first;
+    switch (L.ops) {
+        case OPS_EVAL:
+            hipLaunchKernelGGL((k_batch_thread<NN, OPS_EVAL>), grid, blk, 0, s, c->d, tl, L.count, L.tiles, tile, counter,
+                               c->d_status);
+            break;
+        case OPS_CMP:
+            hipLaunchKernelGGL((k_batch_thread<NN, OPS_CMP>), grid, blk, 0, s, c->d, tl, L.count, L.tiles, tile, counter,
+                               c->d_status);
+            break;
+        default:
+            hipLaunchKernelGGL((k_batch_thread<NN, OPS_MPINT>), grid, blk, 0, s, c->d, tl, L.count, L.tiles, tile,
+                               counter, c->d_status);
+            break;
+    }
+}
+
 grns_status enqueue_plan(grns_ctx *c, const TaskPlan &P, const BatchTask *d_tasks, unsigned long long *d_counters,
                          const grns_batch_config &cfg, int cus, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, d_counters);
     dim3 grid((unsigned)(cus * cfg.blocks_per_cu)), blk(BLOCK);
-    if (P.nthr) {
+    for (size_t li = 0; li < P.lists.size(); li++) {
+        unsigned long long *counter = d_counters + li;
         switch (c->n) {
-#define GRNS_BT(NN)                                                                                          \
-    case NN:                                                                                                 \
-        if (P.thr_ops == OPS_MPINT)                                                                          \
-            hipLaunchKernelGGL((k_batch_thread<NN, OPS_MPINT>), grid, blk, 0, s, c->d, d_tasks, P.nthr,        \
-                               P.thr_tiles, cfg.tile, d_counters, c->d_status);                              \
-        else if (P.thr_ops == OPS_EVALCMP)                                                                   \
-            hipLaunchKernelGGL((k_batch_thread<NN, OPS_EVALCMP>), grid, blk, 0, s, c->d, d_tasks, P.nthr,      \
-                               P.thr_tiles, cfg.tile, d_counters, c->d_status);                              \
-        else                                                                                                 \
-            hipLaunchKernelGGL((k_batch_thread<NN, OPS_EVALCMP | OPS_MPINT>), grid, blk, 0, s, c->d, d_tasks,  \
-                               P.nthr, P.thr_tiles, cfg.tile, d_counters, c->d_status);                      \
-        break;
-            GRNS_BT(4)
-            GRNS_BT(8)
-            GRNS_BT(16)
-            GRNS_BT(32)
-            GRNS_BT(64)
-#undef GRNS_BT
-            default:
-                return GRNS_ERR_UNSUPPORTED;
+            case 4: launch_thread_list<4>(c, P.lists[li], d_tasks, counter, cfg.tile, grid, s); break;
+            case 8: launch_thread_list<8>(c, P.lists[li], d_tasks, counter, cfg.tile, grid, s); break;
+            case 16: launch_thread_list<16>(c, P.lists[li], d_tasks, counter, cfg.tile, grid, s); break;
+            case 32: launch_thread_list<32>(c, P.lists[li], d_tasks, counter, cfg.tile, grid, s); break;
+            case 64: launch_thread_list<64>(c, P.lists[li], d_tasks, counter, cfg.tile, grid, s); break;
+            default: return GRNS_ERR_UNSUPPORTED;
         }
     }
     if (P.ngrp) {
         // group width: the word division's when it has one (the group tasks are mostly divisions)
         const int G = words_group(c) ? words_group(c) : groups_for(c->n);
         hipLaunchKernelGGL(k_batch_group, grid, blk, 0, s, c->d, G, d_tasks + P.nthr, P.ngrp,
-                           P.grp_tiles, (int)P.grp_tile, d_counters + 1, c->d_status);
+                           P.grp_tiles, (int)P.grp_tile, d_counters + BATCH_COUNTERS - 1, c->d_status);
     }
     return hipGetLastError() == hipSuccess ? GRNS_OK : GRNS_ERR_HIP;
 }
@@ -401,7 +444,7 @@ grns_status grns_batch_create(grns_ctx *ctx, const grns_batch_task *tasks, int n
     if (hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) b->cus = 256;
     if (!thread_kernel_available(ctx->n)) b->cfg.threads_per_number = 0;
     if (hipMalloc(&b->d_tasks, sizeof(BatchTask) * b->cap) != hipSuccess ||
-        hipMalloc(&b->d_counters, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc(&b->d_counters, BATCH_COUNTERS * sizeof(unsigned long long)) != hipSuccess) {
         grns_batch_destroy(b);
         return GRNS_ERR_OOM;
     }
@@ -467,7 +510,7 @@ grns_status grns_batch_tune(grns_batch *b, void *stream, int reps, grns_batch_co
     grns_batch_config keep = b->cfg, top = b->cfg;
     double top_ms = -1;
     const int tiles[] = {256, 1024, 4096, 16384};
-    const int bpcs[] = {2, 4, 8};
+    const int bpcs[] = {1, 2, 4, 8};
     std::vector<int> maps;
     bool any_elem = false;
     for (const auto &u : b->user) any_elem |= elementwise(u.op);
@@ -768,7 +811,7 @@ grns_status grns_serve_create(int device, const grns_serve_config *cfg, grns_ser
             hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
             hipMalloc(&sl.d_tasks, sizeof(BatchTask) * sl.cap) != hipSuccess ||
             hipHostMalloc(&sl.h_stage, sizeof(BatchTask) * sl.cap) != hipSuccess ||
-            hipMalloc(&sl.d_counters, 2 * sizeof(unsigned long long)) != hipSuccess ||
+            hipMalloc(&sl.d_counters, BATCH_COUNTERS * sizeof(unsigned long long)) != hipSuccess ||
             hipMalloc(&sl.max_ws, sl.max_ws_bytes) != hipSuccess) {
             grns_serve_destroy(S);
             return GRNS_ERR_OOM;

@@ -38,7 +38,10 @@ struct grns_ctx {
         long long launches = 0;
     };
     bool timing = false;
+    bool stream_tiles = false;      // eval: LDS-staged tiles where the layout allows (test hook, off)
     bool reference_loops = false;  // division and scaling: the reference loops for every number (test hook)
+    int cus = 0;                   // compute units of the device
+    int grid_cap = 0;              // streaming kernels: at most grid_cap x cus blocks (0: one per BLOCK numbers)
     StageTimer timers[3];
     std::vector<int> pow2;
     std::vector<int> mrc;

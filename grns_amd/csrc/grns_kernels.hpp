@@ -140,6 +140,22 @@ GRNS_D void load_number_mm(const int *__restrict__ x, long long smod, long long 
     }
 }
 
+// The store counterpart of load_number_mm (digit i of number j at x[i * smod + j], j < 2^30): the
+// row pointer advances by a scalar add per digit instead of N row offsets i * smod computed ahead of
+// the caller's loop, which would take 2N scalar registers for as long as the loop runs.
+typedef __attribute__((address_space(1))) int global_int_w;
+template <int N>
+GRNS_D void store_number_mm(int *__restrict__ x, long long smod, long long j, const int (&xr)[N]) {
+    const unsigned off = lane_offset((unsigned)j * 4u);
+    const long long step = smod * 4;
+    long long row = reinterpret_cast<long long>(x);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        __builtin_nontemporal_store(xr[i], reinterpret_cast<global_int_w *>(row + off));
+        row = uniform_point(row + step);
+    }
+}
+
 template <int N, int MODE = LD_AUTO>
 GRNS_D void load_number(const int *__restrict__ x, long long smod, long long snum, long long j, int (&xr)[N]) {
     if (MODE == LD_MM || (MODE == LD_AUTO && snum == 1 && j < (1ll << 29))) {
@@ -220,13 +236,14 @@ __device__ __attribute__((noinline)) void coop_eval_lanes(const DevCtx *c, int G
 }
 
 // One evaluation per lane (number j, valid when j < count); all 64 lanes of the wave call it.
-template <int N, bool FAST>
+// MODE LD_MM: the caller guarantees snum == 1 and count <= 2^29.
+template <int N, bool FAST, int MODE = LD_AUTO>
 GRNS_D void eval_step(const DevCtx *__restrict__ c, const int *__restrict__ x, long long smod, long long snum,
                       long long j, long long count, interval_t *__restrict__ out, int *__restrict__ status) {
     bool rare = false;
     if (j < count) {
         int xr[N];
-        load_number<N>(x, smod, snum, j, xr);
+        load_number<N, MODE>(x, smod, snum, j, xr);
         interval_t r;
         if (eval_hot<N, FAST>(c, xr, r)) store_interval(out + j, r);
         else rare = true;
@@ -256,6 +273,93 @@ __global__ __launch_bounds__(BLOCK) void k_eval(const DevCtx *__restrict__ c, co
             mk.low.frac = __builtin_nan(""); mk.low.exp = EVAL_MARK;
             mk.upp.frac = __builtin_nan(""); mk.upp.exp = EVAL_MARK;
             store_interval(out + j, mk);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- LDS-staged streaming
+// The streaming kernels above load a number's residues into registers and then spend most of
+// their time on its arithmetic; while a wave computes it has nothing in flight, and the loads of the
+// few other waves a CU holds do not keep HBM busy.  The tiled forms stage tiles of BLOCK numbers
+// (N rows of BLOCK residues, row-major in LDS) with asynchronous global->LDS loads
+// (global_load_lds_dwordx4: each lane moves 16 bytes, the wave's 64 lanes one contiguous 1 KiB row
+// segment into consecutive LDS bytes) and load tile k + 1 while tile k is computed from LDS.
+// Two buffers are named separately and the loop is unrolled by two, so that no LDS read can alias
+// the buffer being filled.  Conditions (checked by the launcher): modulus-major rows that are
+// 16-byte aligned (x and mod_stride * 4 multiples of 16).
+typedef __attribute__((address_space(3))) void lds_void;
+typedef const __attribute__((address_space(1))) void glob_void;
+
+template <int N>
+GRNS_D void tile_issue(const int *__restrict__ x, long long smod, long long t, int *buf) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int *base = x + t * BLOCK + lane * 4;
+#pragma unroll
+    for (int r = wave; r < N; r += BLOCK / 64)
+        __builtin_amdgcn_global_load_lds((glob_void *)(base + (long long)r * smod), (lds_void *)(buf + r * BLOCK), 16,
+                                         0, 0);
+}
+
+// Every staged load of this wave done (and its earlier stores), then every wave of the block.
+GRNS_D void tile_wait() {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+}
+
+template <int N>
+GRNS_D void tile_read(const int *buf, int (&xr)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; i++) xr[i] = buf[i * BLOCK + threadIdx.x];
+}
+
+template <int N, bool FAST>
+GRNS_D void eval_one(const DevCtx *__restrict__ c, const int (&xr)[N], long long j, interval_t *__restrict__ out,
+                     const RareQ &q) {
+    interval_t r;
+    bool rare = false;
+    if (eval_hot<N, FAST>(c, xr, r)) store_interval(out + j, r);
+    else rare = true;
+    if (!rare_push(q, rare, j)) {
+        interval_t mk;
+        mk.low.frac = __builtin_nan(""); mk.low.exp = EVAL_MARK;
+        mk.upp.frac = __builtin_nan(""); mk.upp.exp = EVAL_MARK;
+        store_interval(out + j, mk);
+    }
+}
+
+// rns_eval_compute[_fast] of count numbers: `tiles` = count / BLOCK full tiles through LDS, the
+// remainder by the last block with direct loads.  The rare queue and its pass as k_eval.
+template <int N, bool FAST>
+__global__ __launch_bounds__(BLOCK) void k_eval_tiles(const DevCtx *__restrict__ c, const int *__restrict__ x,
+                                                      long long smod, long long count, interval_t *__restrict__ out,
+                                                      RareQ q) {
+    __shared__ int bufA[N * BLOCK];
+    __shared__ int bufB[N * BLOCK];
+    const long long tiles = count / BLOCK;
+    long long t = blockIdx.x;
+    if (t < tiles) tile_issue<N>(x, smod, t, bufA);
+    while (t < tiles) {
+        int xr[N];
+        tile_wait();
+        long long tn = t + gridDim.x;
+        if (tn < tiles) tile_issue<N>(x, smod, tn, bufB);
+        tile_read<N>(bufA, xr);
+        eval_one<N, FAST>(c, xr, t * BLOCK + threadIdx.x, out, q);
+        if (tn >= tiles) break;
+        t = tn;
+        tile_wait();
+        tn = t + gridDim.x;
+        if (tn < tiles) tile_issue<N>(x, smod, tn, bufA);
+        tile_read<N>(bufB, xr);
+        eval_one<N, FAST>(c, xr, t * BLOCK + threadIdx.x, out, q);
+        t = tn;
+    }
+    if (blockIdx.x == gridDim.x - 1) {  // the last count % BLOCK numbers
+        const long long j = tiles * BLOCK + threadIdx.x;
+        if (j < count) {
+            int xr[N];
+            load_number<N, LD_MM>(x, smod, 1, j, xr);
+            eval_one<N, FAST>(c, xr, j, out, q);
         }
     }
 }
@@ -541,7 +645,8 @@ GRNS_D void coop_cmp_lanes(const DevCtx *c, const int *__restrict__ x, long long
 }
 
 // One comparison per lane (number j, valid when j < count); all 64 lanes of the wave call it.
-template <int N>
+// MODE LD_MM: the caller guarantees xsn == ysn == 1 and count <= 2^29.
+template <int N, int MODE = LD_AUTO>
 GRNS_D void cmp_step(const DevCtx *__restrict__ c, const int *__restrict__ x, long long xsm, long long xsn,
                      const int *__restrict__ y, long long ysm, long long ysn, long long j, long long count,
                      int *__restrict__ sign, int *__restrict__ status) {
@@ -549,8 +654,8 @@ GRNS_D void cmp_step(const DevCtx *__restrict__ c, const int *__restrict__ x, lo
         bool rare = false;
         if (j < count) {
             int xr[N], yr[N];
-            load_number<N>(x, xsm, xsn, j, xr);
-            load_number<N>(y, ysm, ysn, j, yr);
+            load_number<N, MODE>(x, xsm, xsn, j, xr);
+            load_number<N, MODE>(y, ysm, ysn, j, yr);
             int res = cmp_lane<N>(c, xr, yr, x + j * xsn, xsm, y + j * ysn, ysm);
             rare = (res == 2);
             if (!rare) sign[j] = res;
@@ -1694,8 +1799,12 @@ __global__ __launch_bounds__(BLOCK) void k_mrc_lane(const DevCtx *__restrict__ c
             for (int i = 0; i < N; i++) canon &= (unsigned)xr[i] < (unsigned)cm[i];
             if (canon) {
                 lane_mrc<N>(c, tab, xr);
+                if (dsn == 1 && j < (1ll << 29)) {
+                    store_number_mm<N>(d, dsm, j, xr);
+                } else {
 #pragma unroll
-                for (int i = 0; i < N; i++) d[j * dsn + (long long)i * dsm] = xr[i];
+                    for (int i = 0; i < N; i++) d[j * dsn + (long long)i * dsm] = xr[i];
+                }
             } else {
                 coop = true;
             }
@@ -1920,12 +2029,14 @@ GRNS_D int mpint_digit(int a, int b, int sx, int sy, int m, double dinv) {
     }
 }
 
-// Thread per number: element j (j < count).  OP: 0 add, 1 sub, 2 mul.
-template <int N, int OP>
+// Thread per number: element j (j < count).  OP: 0 add, 1 sub, 2 mul.  MM: the caller guarantees
+// count <= 2^29, so that every digit row is addressed by a 32-bit lane offset from a scalar row
+// pointer (no per-digit 64-bit row offsets held across the caller's loop).
+template <int N, int OP, bool MM = false>
 GRNS_D void mpint_step(const DevCtx *__restrict__ c, const MpintArgs &A, long long j) {
     int a[N], b[N], r[N];
-    load_number<N>(A.xd, A.ld, 1, j, a);
-    load_number<N>(A.yd, A.ld, 1, j, b);
+    load_number<N, MM ? LD_MM : LD_AUTO>(A.xd, A.ld, 1, j, a);
+    load_number<N, MM ? LD_MM : LD_AUTO>(A.yd, A.ld, 1, j, b);
     int sx = A.xs[j], sy = A.ys[j];
     if (OP == 1) sy ^= 1;
     er_t x0 = load_er(A.xe + 2 * j), x1 = load_er(A.xe + 2 * j + 1);
@@ -1949,8 +2060,12 @@ GRNS_D void mpint_step(const DevCtx *__restrict__ c, const MpintArgs &A, long lo
         }
     }
     int sr = (OP == 2) ? ((nz == 0) ? 0 : (sx ^ sy)) : minus;
+    if (MM || j < (1ll << 29)) {
+        store_number_mm<N>(A.rd, A.ld, j, r);
+    } else {
 #pragma unroll
-    for (int i = 0; i < N; i++) __builtin_nontemporal_store(r[i], A.rd + (long long)i * A.ld + j);
+        for (int i = 0; i < N; i++) __builtin_nontemporal_store(r[i], A.rd + (long long)i * A.ld + j);
+    }
     A.rs[j] = sr;
     store_er(A.re + 2 * j, r0);
     store_er(A.re + 2 * j + 1, r1);
@@ -2054,10 +2169,10 @@ __global__ __launch_bounds__(BLOCK) void k_mpint_aos(const DevCtx *__restrict__ 
     }
 }
 
-template <int N, int OP>
+template <int N, int OP, bool MM>
 __global__ __launch_bounds__(BLOCK) void k_mpint(const DevCtx *__restrict__ c, MpintArgs A, long long count) {
     for (long long j = (long long)blockIdx.x * BLOCK + threadIdx.x; j < count; j += (long long)gridDim.x * BLOCK)
-        mpint_step<N, OP>(c, A, j);
+        mpint_step<N, OP, MM>(c, A, j);
 }
 
 // mpint_div (src/mpint.cuh:286-296, CPU path) of element j by a group: rns_div_fast of the

@@ -13,6 +13,7 @@
 #include "../../include/grns.h"
 #include "../../include/grns_debug.h"
 #include "grns_internal.hpp"
+#include "grns_serial.hpp"
 
 using namespace grns;
 
@@ -206,10 +207,45 @@ grns_status build_constants(grns_ctx *c, const int *moduli, int n) {
 
 hipStream_t S(void *s) { return (hipStream_t)s; }
 
+// The LDS-staged streaming kernels: for n = 16 and 32 (two tiles of n x BLOCK residues fit the
+// LDS of a CU twice over), modulus-major rows aligned to 16 bytes, at least one full tile.
+bool tiles_ok(const grns_ctx *c, int n, const int *x, long long smod, size_t count) {
+    return c->stream_tiles && (n == 16 || n == 32) && ((uintptr_t)x & 15) == 0 && (smod & 3) == 0 &&
+           count >= (size_t)BLOCK;
+}
+
+// Their grid: the blocks resident at once on the device (occupancy x CUs), or fewer when there are
+// fewer tiles.
+long long tiles_grid(const grns_ctx *c, size_t count, const void *kernel) {
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, BLOCK, 0) != hipSuccess || per < 1) per = 1;
+    long long tiles = (long long)(count / BLOCK);
+    long long g = (long long)cus * per;
+    return g < tiles ? g : (tiles > 0 ? tiles : 1);
+}
+
+// Grid of the thread-per-number kernels of the large moduli sets (n > 64): blocks of one wave,
+// at most `cap` of them (their private arrays take up to a few KiB per thread).
+long long grid_serial(size_t count, long long cap = 4096) {
+    long long b = (long long)((count + SERIAL_BLOCK - 1) / SERIAL_BLOCK);
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return b;
+}
+
 long long grid_elems(size_t count) {
     long long b = (long long)((count + BLOCK - 1) / BLOCK);
     if (b < 1) b = 1;
     if (b > (1ll << 20)) b = 1ll << 20;
+    return b;
+}
+// The grid of a per-number streaming kernel (grid-stride loops): one block per BLOCK numbers, or at
+// most grid_cap blocks per CU when the context sets a cap.
+long long grid_stream(const grns_ctx *c, size_t count) {
+    long long b = grid_elems(count);
+    if (c->grid_cap > 0 && c->cus > 0 && b > (long long)c->grid_cap * c->cus) b = (long long)c->grid_cap * c->cus;
     return b;
 }
 long long grid_groups(size_t count, int G) {
@@ -379,6 +415,8 @@ grns_status grns_ctx_create(const int *moduli, int n, int device, grns_ctx **out
         return GRNS_OK;
     }
     DeviceGuard guard(device);
+    if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c->cus < 1)
+        c->cus = 256;
     if (hipMalloc(&c->d, sizeof(DevCtx)) != hipSuccess ||
         hipMalloc(&c->d_crt, sizeof(unsigned) * c->crt.size()) != hipSuccess ||
         hipMalloc(&c->d_pow2, sizeof(int) * c->pow2.size()) != hipSuccess ||
@@ -481,7 +519,7 @@ grns_status grns_eval(grns_ctx *c, const int *x, long long smod, long long snum,
     if (count == 0) return GRNS_OK;
     DeviceGuard guard(c->device);
     interval_t *o = reinterpret_cast<interval_t *>(out);
-    dim3 grid((unsigned)grid_elems(count)), blk(BLOCK);
+    dim3 grid((unsigned)grid_stream(c, count)), blk(BLOCK);
     long long cnt = (long long)count;
     RareQ q;
     const bool mm = snum == 1 && count <= ((size_t)1 << 29);
@@ -491,7 +529,12 @@ grns_status grns_eval(grns_ctx *c, const int *x, long long smod, long long snum,
         if (qs != GRNS_OK) return qs;                                                                            \
         dim3 rg((unsigned)rare_grid(count));                                                                     \
         int tk = stage_begin(c, 0, stream);                                                                      \
-        if (mm) {                                                                                                \
+        if (mm && tiles_ok(c, NN, x, smod, count)) {                                                             \
+            dim3 tg((unsigned)tiles_grid(c, count, fast ? (const void *)k_eval_tiles<NN, true>                   \
+                                                        : (const void *)k_eval_tiles<NN, false>));                \
+            if (fast) hipLaunchKernelGGL((k_eval_tiles<NN, true>), tg, blk, 0, S(stream), c->d, x, smod, cnt, o, q); \
+            else hipLaunchKernelGGL((k_eval_tiles<NN, false>), tg, blk, 0, S(stream), c->d, x, smod, cnt, o, q);  \
+        } else if (mm) {                                                                                         \
             if (fast) hipLaunchKernelGGL((k_eval<NN, true, LD_MM>), grid, blk, 0, S(stream), c->d, x, smod, snum,  \
                                          cnt, o, q);                                                             \
             else hipLaunchKernelGGL((k_eval<NN, false, LD_MM>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, \
@@ -522,7 +565,14 @@ grns_status grns_eval(grns_ctx *c, const int *x, long long smod, long long snum,
             break;
     }
 #undef GRNS_EVAL_CASE
-    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
+    if (c->n > 64) {
+        dim3 gs((unsigned)grid_serial(count));
+        if (fast) hipLaunchKernelGGL((k_eval_serial<true>), gs, dim3(SERIAL_BLOCK), 0, S(stream), c->d, x, smod, snum,
+                                     cnt, reinterpret_cast<interval_t *>(out), c->d_status);
+        else hipLaunchKernelGGL((k_eval_serial<false>), gs, dim3(SERIAL_BLOCK), 0, S(stream), c->d, x, smod, snum, cnt,
+                                reinterpret_cast<interval_t *>(out), c->d_status);
+        return launch_check();
+    }
     int G = groups_for(c->n);
     dim3 g2((unsigned)grid_groups(count, G));
     if (fast) hipLaunchKernelGGL((k_eval_group<true>), g2, blk, 0, S(stream), c->d, G, x, smod, snum, cnt, o, c->d_status);
@@ -536,7 +586,7 @@ grns_status grns_cmp(grns_ctx *c, const int *x, long long xsm, long long xsn, co
     if (!c || (count && (!x || !y || !sign))) return GRNS_ERR_ARG;
     if (count == 0) return GRNS_OK;
     DeviceGuard guard(c->device);
-    dim3 grid((unsigned)grid_elems(count)), blk(BLOCK);
+    dim3 grid((unsigned)grid_stream(c, count)), blk(BLOCK);
     long long cnt = (long long)count;
     RareQ q;
     const bool mm = xsn == 1 && ysn == 1 && count <= ((size_t)1 << 29);
@@ -568,7 +618,11 @@ grns_status grns_cmp(grns_ctx *c, const int *x, long long xsm, long long xsn, co
             break;
     }
 #undef GRNS_CMP_CASE
-    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
+    if (c->n > 64) {
+        hipLaunchKernelGGL(k_cmp_serial, dim3((unsigned)grid_serial(count)), dim3(SERIAL_BLOCK), 0, S(stream), c->d, x,
+                           xsm, xsn, y, ysm, ysn, cnt, sign, c->d_status);
+        return launch_check();
+    }
     int G = groups_for(c->n);
     dim3 g2((unsigned)grid_groups(count, G));
     hipLaunchKernelGGL(k_cmp_group, g2, blk, 0, S(stream), c->d, G, x, xsm, xsn, y, ysm, ysn, cnt, sign, c->d_status);
@@ -583,8 +637,15 @@ grns_status grns_div(grns_ctx *c, const int *x, const int *d, long long smod, lo
     const bool fast = (flags & GRNS_DIV_FAST) != 0;
     const bool reference = (flags & GRNS_DIV_REFERENCE) != 0 || c->reference_loops;
     if (count == 0) return GRNS_OK;
-    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
     DeviceGuard guard(c->device);
+    if (c->n > 64) {  // the reference loop on one thread per pair (no word form this wide)
+        dim3 gs((unsigned)grid_serial(count, 2048));
+        if (fast) hipLaunchKernelGGL((k_div_serial<true>), gs, dim3(SERIAL_BLOCK), 0, S(stream), c->d, x, d, smod, snum,
+                                     (long long)count, q, r, c->d_status);
+        else hipLaunchKernelGGL((k_div_serial<false>), gs, dim3(SERIAL_BLOCK), 0, S(stream), c->d, x, d, smod, snum,
+                                (long long)count, q, r, c->d_status);
+        return launch_check();
+    }
     int G = groups_for(c->n);
     dim3 grid((unsigned)grid_groups(count, G)), blk(BLOCK);
     long long cnt = (long long)count;
@@ -620,8 +681,12 @@ grns_status grns_mrc(grns_ctx *c, const int *x, long long smod, long long snum, 
     if (c && c->device < 0) return GRNS_ERR_ARG;
     if (!c || (count && (!x || !d))) return GRNS_ERR_ARG;
     if (count == 0) return GRNS_OK;
-    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
     DeviceGuard guard(c->device);
+    if (c->n > 64) {
+        hipLaunchKernelGGL(k_mrc_serial, dim3((unsigned)grid_serial(count)), dim3(SERIAL_BLOCK), 0, S(stream), c->d, x,
+                           smod, snum, (long long)count, d, dsm, dsn);
+        return launch_check();
+    }
     const long long cnt = (long long)count;
     dim3 blk(BLOCK), lgrid((unsigned)grid_elems(count));
 #define GRNS_MRC_CASE(NN)                                                                                         \
@@ -692,8 +757,12 @@ grns_status grns_scale2pow(grns_ctx *c, const int *x, long long smod, long long 
     if (c && c->device < 0) return GRNS_ERR_ARG;
     if (!c || (count && (!x || !D || !y))) return GRNS_ERR_ARG;
     if (count == 0) return GRNS_OK;
-    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
     DeviceGuard guard(c->device);
+    if (c->n > 64) {  // the reference steps on one thread per number
+        hipLaunchKernelGGL(k_scale2pow_serial, dim3((unsigned)grid_serial(count, 2048)), dim3(SERIAL_BLOCK), 0,
+                           S(stream), c->d, x, smod, snum, D, (long long)count, y);
+        return launch_check();
+    }
     int G = groups_for(c->n);
     dim3 grid((unsigned)grid_groups(count, G)), blk(BLOCK);
     const long long cnt = (long long)count;
@@ -742,7 +811,24 @@ static void max_shape(const grns_ctx *c, size_t count, long long &nblocks, long 
 
 // Workspace of the arg-max: the per-block candidates, then a copy of each candidate's residues.
 static size_t max_ws_bytes(const grns_ctx *c, long long nb) {
-    return (size_t)nb * sizeof(MaxCand) + (size_t)nb * c->n * sizeof(int);
+    size_t b = (size_t)nb * sizeof(MaxCand) + (size_t)nb * c->n * sizeof(int);
+    const size_t sb = (size_t)SERIAL_MAX_BLOCKS * sizeof(SerialCand);  // the serial form's candidates
+    return b > sb ? b : sb;
+}
+
+// Sizes with arg-max kernels; the others take the serial form (any n up to 256).
+static bool max_kernel_size(int n) { return n == 4 || n == 8 || n == 16 || n == 32 || n == 64; }
+
+static grns_status max_serial(grns_ctx *c, const int *x, long long smod, long long snum, size_t count,
+                              int *out_residues, long long *out_idx, void *workspace, void *stream) {
+    const long long nbs = grid_serial(count, SERIAL_MAX_BLOCKS);
+    SerialCand *part = static_cast<SerialCand *>(workspace);
+    hipLaunchKernelGGL(k_max_serial_partial, dim3((unsigned)nbs), dim3(SERIAL_BLOCK), 0, S(stream), c->d, x, smod, snum,
+                       (long long)count, part);
+    if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;
+    hipLaunchKernelGGL(k_max_serial_final, dim3(1), dim3(SERIAL_BLOCK), 0, S(stream), c->d, x, smod, snum, part,
+                       (int)nbs, out_residues, out_idx);
+    return launch_check();
 }
 
 size_t grns_max_workspace_size(const grns_ctx *c, size_t count) {
@@ -793,6 +879,7 @@ grns_status grns_max(grns_ctx *c, const int *x, long long smod, long long snum, 
     max_shape(c, count, nb, ch);
     if (!workspace || ws_bytes < max_ws_bytes(c, nb)) return GRNS_ERR_ARG;
     DeviceGuard guard(c->device);
+    if (!max_kernel_size(c->n)) return max_serial(c, x, smod, snum, count, out_residues, out_idx, workspace, stream);
     MaxCand *part = reinterpret_cast<MaxCand *>(workspace);
     int *part_res = reinterpret_cast<int *>(part + nb);
     return max_run(c, x, smod, snum, count, out_residues, out_idx, part, part_res, nb, ch, stream, true);
@@ -825,6 +912,7 @@ grns_status grns_max_mrc(grns_ctx *c, const int *x, long long smod, long long sn
     if (!workspace || ws_bytes < max_ws_bytes(c, nb)) return GRNS_ERR_ARG;
     if (nb > MRC_FINAL_MAX) return GRNS_ERR_ARG;  // never: max_shape caps the grid at this bound
     DeviceGuard guard(c->device);
+    if (!max_kernel_size(c->n)) return max_serial(c, x, smod, snum, count, out_residues, out_idx, workspace, stream);
     long long *pidx = reinterpret_cast<long long *>(workspace);
     int *pdig = reinterpret_cast<int *>(pidx + nb);
     dim3 grid((unsigned)nb), blk(BLOCK);
@@ -850,24 +938,55 @@ grns_status grns_max_mrc(grns_ctx *c, const int *x, long long smod, long long sn
     return launch_check();
 }
 
+// mpint of the sets without a thread-per-number kernel (n not in {4, 8, 16, 32, 64}, and n > 64):
+// the thread-per-number serial forms over SoA arrays (or, from grns_mpint_aos, mp_int_t structs).
+static MpSerial mp_serial_soa(const grns_mpint_array *a) {
+    MpSerial m;
+    m.d = a->digits; m.dm = a->ld; m.dn = 1;
+    m.s = a->sign; m.ss = 1;
+    m.e = reinterpret_cast<const unsigned char *>(a->eval); m.es = 2 * (long long)sizeof(grns_er);
+    return m;
+}
+
+static grns_status mpint_serial(grns_ctx *c, int op, const MpSerial &X, const MpSerial &Y, const MpSerial &Z,
+                                size_t count, unsigned char *ovf, void *stream) {
+    dim3 gs((unsigned)grid_serial(count, op == 3 ? 2048 : 4096)), blk(SERIAL_BLOCK);
+    const long long cnt = (long long)count;
+    switch (op) {
+        case 0: hipLaunchKernelGGL((k_mpint_serial<0>), gs, blk, 0, S(stream), c->d, X, Y, Z, ovf, cnt); break;
+        case 1: hipLaunchKernelGGL((k_mpint_serial<1>), gs, blk, 0, S(stream), c->d, X, Y, Z, ovf, cnt); break;
+        case 2: hipLaunchKernelGGL((k_mpint_serial<2>), gs, blk, 0, S(stream), c->d, X, Y, Z, ovf, cnt); break;
+        default: hipLaunchKernelGGL(k_mpint_div_serial, gs, blk, 0, S(stream), c->d, X, Y, Z, cnt, c->d_status); break;
+    }
+    return launch_check();
+}
+
 static grns_status mpint_launch(grns_ctx *c, int op, const grns_mpint_array *x, const grns_mpint_array *y,
                                 const grns_mpint_array *z, size_t count, unsigned char *ovf, void *stream) {
     if (!c || !x || !y || !z || c->device < 0) return GRNS_ERR_ARG;
     if (count == 0) return GRNS_OK;
     if (x->ld != y->ld || x->ld != z->ld || x->ld < (long long)count) return GRNS_ERR_ARG;
     DeviceGuard guard(c->device);
-    dim3 grid((unsigned)grid_elems(count)), blk(BLOCK);
+    dim3 grid((unsigned)grid_stream(c, count)), blk(BLOCK);
     long long cnt = (long long)count;
     MpintArgs A;
     A.xd = x->digits; A.xs = x->sign; A.xe = reinterpret_cast<const er_t *>(x->eval);
     A.yd = y->digits; A.ys = y->sign; A.ye = reinterpret_cast<const er_t *>(y->eval);
     A.rd = z->digits; A.rs = z->sign; A.re = reinterpret_cast<er_t *>(z->eval);
     A.ovf = ovf; A.ld = x->ld;
+    const bool mm = count <= ((size_t)1 << 29);
+#define GRNS_MP_LAUNCH(NN, OPV)                                                                        \
+    if (mm) hipLaunchKernelGGL((k_mpint<NN, OPV, true>), grid, blk, 0, S(stream), c->d, A, cnt);       \
+    else hipLaunchKernelGGL((k_mpint<NN, OPV, false>), grid, blk, 0, S(stream), c->d, A, cnt);
 #define GRNS_MP_CASE(NN)                                                                               \
     case NN:                                                                                           \
-        if (op == 0) hipLaunchKernelGGL((k_mpint<NN, 0>), grid, blk, 0, S(stream), c->d, A, cnt);      \
-        else if (op == 1) hipLaunchKernelGGL((k_mpint<NN, 1>), grid, blk, 0, S(stream), c->d, A, cnt); \
-        else hipLaunchKernelGGL((k_mpint<NN, 2>), grid, blk, 0, S(stream), c->d, A, cnt);              \
+        if (op == 0) {                                                                                 \
+            GRNS_MP_LAUNCH(NN, 0)                                                                      \
+        } else if (op == 1) {                                                                          \
+            GRNS_MP_LAUNCH(NN, 1)                                                                      \
+        } else {                                                                                       \
+            GRNS_MP_LAUNCH(NN, 2)                                                                      \
+        }                                                                                              \
         return launch_check();
     switch (c->n) {
         GRNS_MP_CASE(4)
@@ -876,9 +995,10 @@ static grns_status mpint_launch(grns_ctx *c, int op, const grns_mpint_array *x, 
         GRNS_MP_CASE(32)
         GRNS_MP_CASE(64)
         default:
-            return GRNS_ERR_UNSUPPORTED;
+            return mpint_serial(c, op, mp_serial_soa(x), mp_serial_soa(y), mp_serial_soa(z), count, ovf, stream);
     }
 #undef GRNS_MP_CASE
+#undef GRNS_MP_LAUNCH
 }
 
 grns_status grns_mpint_add(grns_ctx *c, const grns_mpint_array *x, const grns_mpint_array *y,
@@ -899,8 +1019,8 @@ grns_status grns_mpint_div(grns_ctx *c, const grns_mpint_array *x, const grns_mp
     if (!c || !x || !y || !z || c->device < 0) return GRNS_ERR_ARG;
     if (count == 0) return GRNS_OK;
     if (x->ld != y->ld || x->ld != z->ld || x->ld < (long long)count) return GRNS_ERR_ARG;
-    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
     DeviceGuard guard(c->device);
+    if (c->n > 64) return mpint_serial(c, 3, mp_serial_soa(x), mp_serial_soa(y), mp_serial_soa(z), count, nullptr, stream);
     MpintArgs A;
     A.xd = x->digits; A.xs = x->sign; A.xe = reinterpret_cast<const er_t *>(x->eval);
     A.yd = y->digits; A.ys = y->sign; A.ye = reinterpret_cast<const er_t *>(y->eval);
@@ -934,8 +1054,18 @@ grns_status grns_mpint_aos(grns_ctx *c, int op, const void *x, const void *y, vo
     if (elem_bytes != grns_mpint_aos_bytes(c)) return GRNS_ERR_ARG;  // a different RNS_MODULI_SIZE
     if (count == 0) return GRNS_OK;
     if (!x || !y || !z) return GRNS_ERR_ARG;
-    if (c->n > 64) return GRNS_ERR_UNSUPPORTED;
     DeviceGuard guard(c->device);
+    if (c->n > 64) {  // the same structs through the serial forms: digits, sign, eval by strides
+        auto view = [&](const void *base) {
+            MpSerial m;
+            m.d = static_cast<const int *>(base); m.dm = 1; m.dn = (long long)elem_bytes / 4;
+            m.s = static_cast<const int *>(base) + c->n; m.ss = (long long)elem_bytes / 4;
+            m.e = static_cast<const unsigned char *>(base) + mpint_aos_eoff(c->n); m.es = (long long)elem_bytes;
+            return m;
+        };
+        const int sop = op == GRNS_OP_MPINT_ADD ? 0 : op == GRNS_OP_MPINT_SUB ? 1 : op == GRNS_OP_MPINT_MUL ? 2 : 3;
+        return mpint_serial(c, sop, view(x), view(y), view(z), count, sop == 3 ? nullptr : ovf, stream);
+    }
     MpAos A;
     A.x = static_cast<const unsigned char *>(x);
     A.y = static_cast<const unsigned char *>(y);
@@ -1138,6 +1268,18 @@ grns_status grns_debug_stage_read(grns_ctx *c, int op, double *ms_total, long lo
     *launches = t.launches;
     t.ms = 0.0;
     t.launches = 0;
+    return GRNS_OK;
+}
+
+grns_status grns_debug_grid_cap(grns_ctx *c, int blocks_per_cu) {
+    if (!c || blocks_per_cu < 0) return GRNS_ERR_ARG;
+    c->grid_cap = blocks_per_cu;
+    return GRNS_OK;
+}
+
+grns_status grns_debug_stream_tiles(grns_ctx *c, int on) {
+    if (!c) return GRNS_ERR_ARG;
+    c->stream_tiles = on != 0;
     return GRNS_OK;
 }
 

@@ -185,27 +185,41 @@ class Synth:
 
 
 # ---------------------------------------------------------------------- multi-request workloads
+ALIGN = 64  # elements: 256 bytes of int32 digits
+
+
 def mpint_batch(ctx, syn, vectors=4096, max_len=65536, seed=4, bits=480):
     """The batched mpint workload: `vectors` vectors with lengths floor(e^U[0, ln max_len]) (log-uniform
-    in [1, max_len]) laid end to end, an operation (0 add, 1 sub, 2 mul) drawn per vector, operands
-    X and Y of uniform `bits`-bit magnitudes with random signs and their evaluations (ctx.eval) as the
-    eval fields.  Returns a dict: X, Y ((digits (n, total), sign (total,), eval (total, 2, 2))), lens,
-    offs, ops (numpy arrays) and total."""
+    in [1, max_len]), an operation (0 add, 1 sub, 2 mul) drawn per vector, operands X and Y of uniform
+    `bits`-bit magnitudes with random signs and their evaluations (ctx.eval) as the eval fields.
+
+    The vectors share one digit-major array per operand, one after another, each starting on a
+    256-byte boundary (a multiple of ALIGN elements), and the row stride `span` is a multiple of
+    ALIGN too, so that every digit row of every vector starts on a 256-byte boundary: with rows at
+    unaligned offsets each wave's 256-byte row segment spans three 128-byte lines instead of two, and
+    an elementwise pass over 2.5e7 elements runs at 3.95 instead of 5.33 TB/s on MI355X.  The padding
+    elements between vectors hold valid numbers and belong to no vector.
+
+    Returns a dict: X, Y ((digits (n, span), sign (span,), eval (span, 2, 2))), lens, offs, ops
+    (numpy arrays), total (elements in vectors) and span."""
     import torch
     rng = np.random.default_rng(seed)
     lens = np.floor(np.exp(rng.uniform(0, np.log(max_len), size=vectors))).astype(np.int64).clip(1, max_len)
     ops = rng.integers(0, 3, size=vectors)
     total = int(lens.sum())
-    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    slots = (lens + ALIGN - 1) // ALIGN * ALIGN
+    offs = np.concatenate([[0], np.cumsum(slots)[:-1]]).astype(np.int64)
+    span = int(slots.sum())
 
     def operand(stream):
-        b = torch.full((total,), bits, dtype=torch.int32, device=syn.device)
+        b = torch.full((span,), bits, dtype=torch.int32, device=syn.device)
         d = syn.bits(b, seed, stream, top_set=False)
-        s = syn.uniform_u32(total, 0, 1, seed, stream + 100)
+        s = syn.uniform_u32(span, 0, 1, seed, stream + 100)
         ev = ctx.eval(d)  # interval records have the mp_int_t eval[2] layout
-        return d, s, ev.view(total, 2, 2)
+        return d, s, ev.view(span, 2, 2)
 
-    return {"X": operand(10), "Y": operand(20), "lens": lens, "offs": offs, "ops": ops, "total": total}
+    return {"X": operand(10), "Y": operand(20), "lens": lens, "offs": offs, "ops": ops, "total": total,
+            "span": span}
 
 
 MPINT_OPS = ["mpint_add", "mpint_sub", "mpint_mul"]

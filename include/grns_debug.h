@@ -34,6 +34,15 @@ grns_status grns_debug_stage_read(grns_ctx *ctx, int op, double *ms_total, long 
  * the reference's loops (src/rnsdiv.cuh, src/rnsscal.cuh) for every number instead of the word
  * forms, so that the two can be compared on the device. */
 grns_status grns_debug_reference_loops(grns_ctx *ctx, int on);
+/* Test hook: on = 1 makes grns_eval (n = 16 and 32, modulus-major rows aligned to 16 bytes) stage
+ * its residues through double-buffered LDS tiles instead of loading them straight into registers,
+ * the default.  Both forms give the same results; the tiled one is slower on gfx950 (0.58 against
+ * 0.49 ms for 2^24 numbers at n = 32: its 185 VGPRs halve the waves per SIMD). */
+grns_status grns_debug_stream_tiles(grns_ctx *ctx, int on);
+/* Test hook: caps the grid of the per-number streaming kernels of grns_eval, grns_cmp and the mpint
+ * operations (grid-stride loops) at blocks_per_cu blocks per compute unit; 0 restores one block per
+ * 256 numbers.  Results do not depend on it; the hook lets grid shapes be timed on the device. */
+grns_status grns_debug_grid_cap(grns_ctx *ctx, int blocks_per_cu);
 /* Scheduler test hook: a server whose launches run nothing and complete delay_us after issue;
  * it accepts host-only contexts (device -1), so admission and batching run without a GPU. */
 grns_status grns_serve_create_sim(const grns_serve_config *cfg, int delay_us, grns_server **out);

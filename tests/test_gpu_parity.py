@@ -5,7 +5,7 @@ import random
 import numpy as np
 import pytest
 
-from tests.helpers import SETS, cmp_pairs, div_inputs, eval_inputs, to_res
+from tests.helpers import SETS, cmp_pairs, div_inputs, eval_inputs, from_res, to_res
 
 pytestmark = pytest.mark.gpu
 
@@ -53,6 +53,36 @@ def test_eval_parity(ctxs, dev, n, fast):
     bad = np.nonzero((g != r).any(axis=1))[0]
     assert bad.size == 0, f"{bad.size} mismatches, first value {vals[bad[0]]}"
     assert c.status() == 0
+
+
+@pytest.mark.parametrize("n", [16, 32])
+@pytest.mark.parametrize("fast", [False, True])
+def test_eval_lds_tiles_form(dev, n, fast):
+    """The LDS-staged eval (a test hook, off by default; taken for 16-byte aligned rows whose
+    stride is a multiple of 4) equals the oracle: whole tiles and a ragged last one, with a packed
+    row stride and with rows padded to a wider stride."""
+    import grns_amd as G
+    from oracle.oracle import Oracle
+    mod = G.largest_primes_below(n)
+    c, o = G.Context(mod, 0), Oracle(mod)
+    try:
+        c.stream_tiles(True)
+        vals = eval_inputs(mod, 5 * 256 + 76, seed=40 + n)
+        x = to_res(vals, mod)
+        got = c.eval(_dev(x, dev), fast=fast)
+        assert got.cpu().numpy().tobytes() == o.eval_array(x, fast=fast).tobytes()
+        wide = np.zeros((n, x.shape[1] + 12), np.int32)
+        wide[:, :x.shape[1]] = x
+        ref = o.eval_array(x, fast=fast)
+        got = c.eval(_dev(wide, dev)[:, :x.shape[1]], fast=fast)
+        assert got.cpu().numpy().tobytes() == ref.tobytes()
+        # small inputs of a 16-moduli set can need a refinement factor past the reference's table:
+        # the status bit is raised exactly when the oracle met one (NaN bounds there)
+        past_table = np.isnan(ref["low"]["frac"]).any() or np.isnan(ref["upp"]["frac"]).any()
+        assert c.status() == (G.STATUS_REFINE_RANGE if past_table else 0)
+    finally:
+        c.close()
+        o.close()
 
 
 @pytest.mark.parametrize("n", [8, 32])
@@ -831,11 +861,6 @@ def test_shipped_moduli_sets(dev, name):
     assert (c.mrc(xd).cpu().numpy() == np.stack([o.mrc(x[:, j]) for j in range(x.shape[1])], axis=1)).all()
     truth = max(range(len(vals)), key=lambda k: (vals[k], -k))
     for mrc in (False, True):
-        if len(mod) not in (4, 8, 16, 32, 64):  # the arg-max kernels are built for these sizes
-            with pytest.raises(G.GrnsError) as ei:
-                c.max(xd, mrc=mrc)
-            assert ei.value.code == G.GRNS_ERR_UNSUPPORTED
-            continue
         res, idx = c.max(xd, mrc=mrc)
         assert int(idx.item()) == truth and res.cpu().numpy().tolist() == x[:, truth].tolist(), mrc
     if M > 1 << 40:
@@ -864,3 +889,151 @@ def test_shipped_moduli_sets(dev, name):
     assert c.status() == (G.STATUS_REFINE_RANGE if past_table else 0)
     c.close()
     o.close()
+
+
+LARGE_SETS = ["src/params/moduli-64-4096-bit/params.128m.2048b.h", "src/params/moduli-64-4096-bit/params.256m.4096b.h",
+              "src/params/moduli-100-6400-bit/params.128m.3200b.h", "src/params/moduli-100-6400-bit/params.256m.6400b.h"]
+
+
+def _shipped(name):
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "params_sets.json")) as f:
+        return {e["file"]: e["moduli"] for e in json.load(f)}[name]
+
+
+@pytest.mark.parametrize("name", LARGE_SETS)
+def test_large_moduli_sets(dev, name):
+    """128 and 256 moduli (wider than a wave: one thread per number through the CPU path's steps):
+    evaluation, comparison, division (both loops), scaling, mixed-radix digits, both arg-max
+    pipelines, mpint add/sub/mul/div on arrays and on mp_int_t structs, against the oracle bit for bit
+    and the integers."""
+    import grns_amd as G
+    from oracle.oracle import ER_DTYPE, Oracle
+    mod = _shipped(name)
+    n, M = len(mod), G.product(mod)
+    c, o = G.Context(mod, 0), Oracle(mod)
+    rnd = random.Random(n + mod[0] % 97)
+    vals = eval_inputs(mod, 600, seed=3)[:600]
+    x = to_res(vals, mod)
+    xd = _dev(x, dev)
+    for fast in (False, True):
+        assert c.eval(xd, fast=fast).cpu().numpy().tobytes() == o.eval_array(x, fast=fast).tobytes(), fast
+    xs, ys = cmp_pairs(mod, 600, seed=4)
+    a, b = to_res(xs, mod), to_res(ys, mod)
+    assert (c.cmp(_dev(a, dev), _dev(b, dev)).cpu().numpy() == o.cmp_array(a, b)).all()
+    dx, dd = div_inputs(mod, 40, seed=5)
+    p, q = to_res(dx, mod), to_res(dd, mod)
+    for fast in (True, False):
+        gq, gr = c.div(_dev(p, dev), _dev(q, dev), fast=fast)
+        oq, orr = o.div_array(p, q, fast=fast)
+        assert (gq.cpu().numpy() == oq).all() and (gr.cpu().numpy() == orr).all(), fast
+        assert [(v1, v2) for v1, v2 in zip(from_res(gq.cpu().numpy(), mod), from_res(gr.cpu().numpy(), mod))] == \
+            [divmod(u, w) for u, w in zip(dx, dd)], fast
+    D = np.array([rnd.choice([0, 1, 29, 30, 31, rnd.randrange(o.log2M + 3)]) for _ in vals], np.int32)
+    sc = c.scale2pow(xd, _dev(D, dev)).cpu().numpy()
+    assert (sc == o.scale2pow_array(x, D.astype(np.uint32))).all()
+    assert from_res(sc, mod) == [v >> int(k) for v, k in zip(vals, D)]
+    assert (c.mrc(xd).cpu().numpy() == np.stack([o.mrc(x[:, j]) for j in range(x.shape[1])], axis=1)).all()
+    truth = max(range(len(vals)), key=lambda k: (vals[k], -k))
+    xn = x.astype(np.int64)
+    for j in range(0, xn.shape[1], 5):  # non-canonical residues stand for their value mod m_i
+        i = rnd.randrange(n)
+        xn[i, j] += mod[i] if xn[i, j] + mod[i] < 2 ** 31 else -mod[i]
+    for mrc in (False, True):
+        for arr in (x, xn.astype(np.int32)):
+            res, idx = c.max(_dev(arr, dev), mrc=mrc)
+            assert int(idx.item()) == truth and res.cpu().numpy().tolist() == arr[:, truth].tolist(), mrc
+    bits = (M.bit_length() - 6) // 2
+    mv = [rnd.choice([-1, 1]) * rnd.randrange(1, 1 << rnd.randrange(2, bits)) for _ in range(200)]
+    nv = [rnd.choice([-1, 1]) * rnd.randrange(1, 1 << rnd.randrange(2, bits)) for _ in range(200)]
+
+    def side(v):
+        dg = to_res([abs(t) for t in v], mod)
+        sg = np.array([1 if t < 0 else 0 for t in v], np.int32)
+        ev = o.eval_array(dg)
+        e = np.zeros((len(v), 2), ev["low"].dtype)
+        e[:, 0], e[:, 1] = ev["low"], ev["upp"]
+        return (dg, sg, e), (_dev(dg, dev), _dev(sg, dev), _dev(e.view(np.float64).reshape(len(v), 2, 2), dev))
+    (Xn, X), (Yn, Y) = side(mv), side(nv)
+    for op in ("add", "sub", "mul", "div"):
+        Z = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
+        c.mpint(op, X, Y, Z, len(mv))
+        if op == "div":
+            rd, rs, re = o.mpint_div_array(Xn[0], Xn[1], Yn[0], Yn[1])
+        else:
+            rd, rs, re, _ = o.mpint_array(op, *Xn, *Yn)
+        assert (Z[0].cpu().numpy() == rd).all() and (Z[1].cpu().numpy() == rs).all(), op
+        assert Z[2].cpu().numpy().tobytes() == re.tobytes(), op
+    # the same operations on mp_int_t structs (the reference's layout)
+    eoff = (4 * n + 4 + 7) // 8 * 8
+    rec = eoff + 32
+    L = G.lib()
+    assert L.grns_mpint_aos_bytes(c.handle) == rec
+
+    def records(T):
+        r = np.zeros((len(mv), rec), np.uint8)
+        r[:, :4 * n] = np.ascontiguousarray(T[0].T).view(np.uint8).reshape(len(mv), 4 * n)
+        r[:, 4 * n:4 * n + 4] = T[1].view(np.uint8).reshape(-1, 4)
+        r[:, eoff:] = T[2].view(np.uint8).reshape(len(mv), 32)
+        return torch.from_numpy(r).to(dev)
+    RX, RY = records(Xn), records(Yn)
+    for code, op in ((3, "add"), (4, "sub"), (5, "mul"), (10, "div")):
+        RZ = torch.zeros_like(RX)
+        assert L.grns_mpint_aos(c.handle, code, RX.data_ptr(), RY.data_ptr(), RZ.data_ptr(), len(mv), rec, None,
+                                None) == 0
+        z = RZ.cpu().numpy()
+        if op == "div":
+            rd, rs, re = o.mpint_div_array(Xn[0], Xn[1], Yn[0], Yn[1])
+        else:
+            rd, rs, re, _ = o.mpint_array(op, *Xn, *Yn)
+        assert (np.ascontiguousarray(z[:, :4 * n]).view(np.int32).T == rd).all(), op
+        assert (np.ascontiguousarray(z[:, 4 * n:4 * n + 4]).view(np.int32).reshape(-1) == rs).all(), op
+        assert np.ascontiguousarray(z[:, eoff:]).view(ER_DTYPE).reshape(-1, 2).tobytes() == re.tobytes(), op
+    ev = [o.eval_array(x, fast=f) for f in (False, True)]
+    past_table = any(np.isnan(e["low"]["frac"]).any() or np.isnan(e["upp"]["frac"]).any() for e in ev)
+    assert c.status() == (G.STATUS_REFINE_RANGE if past_table else 0)
+    c.close()
+    o.close()
+
+
+@pytest.mark.parametrize("n", [12, 20])
+def test_mpint_and_max_other_sizes(dev, n):
+    """Moduli counts without thread-per-number kernels (n = 12, 20): mpint and the arg-max through the
+    serial forms, against the oracle and the integers."""
+    import grns_amd as G
+    from oracle.oracle import Oracle
+    mod = G.largest_primes_below(n)
+    M = G.product(mod)
+    c, o = G.Context(mod, 0), Oracle(mod)
+    rnd = random.Random(n)
+    vals = [rnd.randrange(M) for _ in range(3000)]
+    vals[1234] = max(vals)
+    x = to_res(vals, mod)
+    truth = max(range(len(vals)), key=lambda k: (vals[k], -k))
+    for mrc in (False, True):
+        res, idx = c.max(_dev(x, dev), mrc=mrc)
+        assert int(idx.item()) == truth and res.cpu().numpy().tolist() == x[:, truth].tolist()
+    bits = (M.bit_length() - 6) // 2
+    mv = [rnd.choice([-1, 1]) * rnd.randrange(1 << rnd.randrange(1, bits)) for _ in range(500)]
+    nv = [rnd.choice([-1, 1]) * rnd.randrange(1, 1 << rnd.randrange(2, bits)) for _ in range(500)]
+
+    def side(v):
+        dg = to_res([abs(t) for t in v], mod)
+        sg = np.array([1 if t < 0 else 0 for t in v], np.int32)
+        ev = o.eval_array(dg)
+        e = np.zeros((len(v), 2), ev["low"].dtype)
+        e[:, 0], e[:, 1] = ev["low"], ev["upp"]
+        return (dg, sg, e), (_dev(dg, dev), _dev(sg, dev), _dev(e.view(np.float64).reshape(len(v), 2, 2), dev))
+    (Xn, X), (Yn, Y) = side(mv), side(nv)
+    for op in ("add", "sub", "mul", "div"):
+        Z = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
+        c.mpint(op, X, Y, Z, len(mv))
+        if op == "div":
+            rd, rs, re = o.mpint_div_array(Xn[0], Xn[1], Yn[0], Yn[1])
+        else:
+            rd, rs, re, _ = o.mpint_array(op, *Xn, *Yn)
+        assert (Z[0].cpu().numpy() == rd).all() and (Z[1].cpu().numpy() == rs).all(), op
+        assert Z[2].cpu().numpy().tobytes() == re.tobytes(), op
+    assert c.status() == 0

@@ -83,9 +83,10 @@ def test_batched_mpint_full_workload(dev):
     X, Y, lens, offs, ops, total = W["X"], W["Y"], W["lens"], W["offs"], W["ops"], W["total"]
     print(f"batched mpint workload: {len(lens)} vectors, {total} elements")
     Zb = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
-    Zp = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
-    ob = torch.full((total,), 7, dtype=torch.uint8, device=dev)
-    op_ = torch.full((total,), 7, dtype=torch.uint8, device=dev)
+    # zero-filled outputs: the padding between vectors belongs to no task and stays 0 in both forms
+    Zp = (torch.zeros_like(X[0]), torch.zeros_like(X[1]), torch.zeros_like(X[2]))
+    ob = torch.full((W["span"],), 7, dtype=torch.uint8, device=dev)
+    op_ = torch.zeros((W["span"],), dtype=torch.uint8, device=dev)
     tasks = [G.make_task(MPINT_OPS[o], int(ln), mx=mpint_view(X, f, ln), my=mpint_view(Y, f, ln),
                          mz=mpint_view(Zb, f, ln), overflow=ob[f:f + ln])
              for o, f, ln in zip(ops, offs, lens)]
