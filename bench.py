@@ -30,6 +30,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 N_MODULI = 32
 COUNT = 1 << 24
+# the list kernel of the batched mpint workload (n = 32, op class mpint) as the profilers name it
+BATCH_KERNEL = "k_batch_thread<32, 4>"
 METRIC = "RNS ops/sec + % HBM roofline (eval/cmp/max, n=32); batched p50/p99 latency"
 
 
@@ -91,8 +93,8 @@ COPY_BYTES = 1 << 31
 def pmc_workload():
     """The program the counter runs profile: a plain 2 GiB device copy (calibration of FETCH_SIZE
     and WRITE_SIZE), then two launches each of arg-max (both pipelines), evaluation and comparison on
-    the headline workload (n = 32, N = 2^24), and of division and scaling on 2^18 pairs of the n = 64
-    workload."""
+    the headline workload (n = 32, N = 2^24), of division and scaling on 2^18 pairs of the n = 64
+    workload, and of the batched mpint workload (4096 vectors, n = 32)."""
     import torch
 
     import grns_amd as G
@@ -134,34 +136,68 @@ def pmc_workload():
         c64.div(x, d, fast=True, q=q, r=r)
         c64.scale2pow(x, D, y=q)
     torch.cuda.synchronize()
+    del x, d, D, q, r
+    torch.cuda.empty_cache()
+    # the batched mpint workload: two launches of its list kernel at the configuration the tuner
+    # picks on MI355X (tiles of 256 numbers, 2 persistent blocks per CU)
+    from grns_amd.workload import MPINT_OPS, mpint_batch, mpint_view
+    W = mpint_batch(ctx, syn)
+    X, Y = W["X"], W["Y"]
+    Z = tuple(torch.empty_like(a) for a in X)
+    tasks = [G.make_task(MPINT_OPS[o], int(ln), mx=mpint_view(X, f, ln), my=mpint_view(Y, f, ln),
+                         mz=mpint_view(Z, f, ln)) for o, f, ln in zip(W["ops"], W["offs"], W["lens"])]
+    batch = G.Batch(ctx, tasks)
+    batch.set_config(tile=256, blocks_per_cu=2, threads_per_number=1, use_graph=False)
+    for _ in range(2):
+        batch.launch()
+    torch.cuda.synchronize()
+    batch.close()
     log("[bench] pmc workload done")
+
+
+def _short_kernel(name):
+    name = name[5:] if name.startswith("void ") else name
+    name = name.split("(")[0] if not name.startswith("(") else name
+    return name.replace("grns::", "")
+
+
+def _counter_rows(d):
+    """Rows of a rocprofv3 counter_collection CSV in dispatch order."""
+    import csv
+    import glob
+    (path,) = glob.glob(os.path.join(d, "*counter_collection.csv"))
+    return sorted(csv.DictReader(open(path)), key=lambda r: int(r["Dispatch_Id"]))
+
+
+def _first_dispatches(rows, counter, per_kernel=2):
+    """{kernel: [value of each of its first `per_kernel` dispatches]} for one counter.  The counter
+    program launches each measured kernel twice on its workload before anything else (input
+    generation of a later workload) dispatches the same kernel again."""
+    seen = {}
+    for r in rows:
+        if r["Counter_Name"] != counter:
+            continue
+        k = _short_kernel(r["Kernel_Name"])
+        vals = seen.setdefault(k, {})
+        did = int(r["Dispatch_Id"])
+        if did in vals or len(vals) < per_kernel:
+            vals[did] = vals.get(did, 0.0) + float(r["Counter_Value"])
+    return {k: list(v.values()) for k, v in seen.items()}
 
 
 def traffic_from_counters(fetch_dir, write_dir, out_path):
     """profiles/traffic.json from the counter_collection CSVs of a FETCH_SIZE run and a WRITE_SIZE
-    run of --pmc-workload: per kernel, the average KiB per dispatch times 1024 times the factor that
-    maps the counter's reading for the 2 GiB copy onto the bytes the copy moves."""
-    import csv
-    import glob
-
-    def short(name):
-        name = name[5:] if name.startswith("void ") else name
-        name = name.split("(")[0] if not name.startswith("(") else name
-        return name.replace("grns::", "")
+    run of --pmc-workload: per kernel, the average KiB over its first two dispatches times 1024
+    times the factor that maps the counter's reading for the 2 GiB copy onto the bytes the copy
+    moves."""
 
     def read(d, counter):
-        (path,) = glob.glob(os.path.join(d, "*counter_collection.csv"))
-        per = {}
-        copies = []
-        for r in csv.DictReader(open(path)):
-            if r["Counter_Name"] != counter:
-                continue
-            v = float(r["Counter_Value"])
-            if r["Kernel_Name"].startswith("__amd_rocclr_copyBuffer"):
-                copies.append((int(r["Grid_Size"]), v))
-            per.setdefault(short(r["Kernel_Name"]), []).append(v)
+        rows = _counter_rows(d)
+        copies = [(int(r["Grid_Size"]), float(r["Counter_Value"])) for r in rows
+                  if r["Counter_Name"] == counter and r["Kernel_Name"].startswith("__amd_rocclr_copyBuffer")]
         big = max(g for g, _ in copies)
         copy_kib = sum(v for g, v in copies if g == big) / sum(1 for g, _ in copies if g == big)
+        per = _first_dispatches(rows, counter)
         return {k: sum(v) / len(v) for k, v in per.items()}, copy_kib
 
     fetch, fcopy = read(fetch_dir, "FETCH_SIZE")
@@ -180,6 +216,31 @@ def traffic_from_counters(fetch_dir, write_dir, out_path):
     with open(out_path, "w") as f:
         json.dump(res, f, indent=1)
     log(f"[bench] wrote {out_path}")
+
+
+SQ_COUNTERS = ("SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES", "SQ_WAVE_CYCLES", "SQ_WAVES", "SQ_INSTS_SALU",
+               "SQ_INSTS_VMEM_RD", "GRBM_GUI_ACTIVE")
+
+
+def sq_summary(sq_dir):
+    """Markdown table of the SQ counters of a --pmc-workload run (the SQ_COUNTERS in one pass), per
+    kernel, averaged over its first two dispatches.  VALU busy = 4 x SQ_ACTIVE_INST_VALU / SIMDs /
+    (GRBM_GUI_ACTIVE / XCDs): a wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles; MI355X
+    has 1024 SIMDs in 8 XCDs, and the GRBM counter sums the XCDs' clocks."""
+    rows = _counter_rows(sq_dir)
+    per = {c: _first_dispatches(rows, c) for c in SQ_COUNTERS}
+    out = ["| Kernel | waves | VALU instr / wave | SALU instr / wave | VMEM loads / wave | GPU cycles | VALU busy |",
+           "|---|---|---|---|---|---|---|"]
+    for k in per["SQ_WAVES"]:
+        if not k.startswith("k_"):
+            continue
+        avg = {c: sum(per[c][k]) / len(per[c][k]) for c in SQ_COUNTERS if k in per[c]}
+        waves = max(avg["SQ_WAVES"], 1.0)
+        cycles = avg["GRBM_GUI_ACTIVE"] / 8
+        busy = 4 * avg["SQ_ACTIVE_INST_VALU"] / 1024 / max(cycles, 1.0)
+        out.append(f"| `{k}` | {waves:.0f} | {avg['SQ_INSTS_VALU'] / waves:.0f} | {avg['SQ_INSTS_SALU'] / waves:.0f} | "
+                   f"{avg['SQ_INSTS_VMEM_RD'] / waves:.1f} | {cycles:.0f} | {100 * busy:.0f}% |")
+    print("\n".join(out))
 
 
 CPU_RUNS = 7  # timed runs per CPU figure (after one untimed run); the median is reported
@@ -329,8 +390,9 @@ def batched_mpint_leg(steps, warmup, vectors=4096, max_len=65536, seed=4):
     ctx = G.Context(mod, 0)
     W = mpint_batch(ctx, Synth(mod), vectors=vectors, max_len=max_len, seed=seed)
     X, Y, lens, offs, ops, total = W["X"], W["Y"], W["lens"], W["offs"], W["ops"], W["total"]
-    Zb = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
-    Zp = (torch.empty_like(X[0]), torch.empty_like(X[1]), torch.empty_like(X[2]))
+    # zero-filled outputs: the padding between vectors belongs to no task and stays 0 in both forms
+    Zb = (torch.zeros_like(X[0]), torch.zeros_like(X[1]), torch.zeros_like(X[2]))
+    Zp = (torch.zeros_like(X[0]), torch.zeros_like(X[1]), torch.zeros_like(X[2]))
     tasks = [G.make_task(MPINT_OPS[op], int(ln), mx=mpint_view(X, o, ln), my=mpint_view(Y, o, ln),
                          mz=mpint_view(Zb, o, ln)) for op, o, ln in zip(ops, offs, lens)]
     batch = G.Batch(ctx, tasks)
@@ -356,7 +418,7 @@ def batched_mpint_leg(steps, warmup, vectors=4096, max_len=65536, seed=4):
                        "add/sub/mul per vector, digit rows 256-byte aligned",
            "elements": total, "batched_ms": round(ms_b, 4), "per_call_ms": round(ms_p, 4),
            "speedup": round(ms_p / ms_b, 2), "elem_ops_per_s": round(total / (ms_b * 1e-3)),
-           "roofline": roofline(bytes_per * total, ms_b, "k_batch_thread"), "tuned": cfg, "bit_exact_vs_per_call": same}
+           "roofline": roofline(bytes_per * total, ms_b, BATCH_KERNEL), "tuned": cfg, "bit_exact_vs_per_call": same}
     batch.close()
     return out
 
@@ -450,7 +512,12 @@ def main():
     ap.add_argument("--pmc-workload", action="store_true", help="run the counter-profiling program and exit")
     ap.add_argument("--traffic", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT"),
                     help="write the traffic summary from two rocprofv3 --pmc runs of --pmc-workload and exit")
+    ap.add_argument("--sq-summary", metavar="SQ_DIR",
+                    help="print the SQ counter table of a rocprofv3 --pmc run of --pmc-workload and exit")
     args = ap.parse_args()
+    if args.sq_summary:
+        sq_summary(args.sq_summary)
+        return
     if args.traffic:
         traffic_from_counters(*args.traffic)
         return
