@@ -84,7 +84,8 @@ DEBUG_EXPORTS = [
     "grns_debug_mulmod_recip_d",
     "grns_debug_mulmod_recip", "grns_debug_addmod_recip", "grns_debug_bn_err", "grns_debug_fast_scalbn",
     "grns_debug_filter_tau", "grns_debug_filter_err", "grns_debug_stage_timing", "grns_debug_stage_read",
-    "grns_debug_reference_loops", "grns_debug_stream_tiles", "grns_debug_grid_cap",
+    "grns_debug_reference_loops", "grns_debug_stream_tiles", "grns_debug_grid_cap", "grns_debug_batch_per_call",
+    "grns_debug_rare_staging",
 ]
 
 
@@ -157,6 +158,8 @@ def lib():
     L.grns_debug_reference_loops.argtypes = [P, I]
     L.grns_debug_stream_tiles.argtypes = [P, I]
     L.grns_debug_grid_cap.argtypes = [P, I]
+    L.grns_debug_rare_staging.argtypes = [P, I]
+    L.grns_debug_batch_per_call.argtypes = [P, P]
     L.grns_debug_stage_read.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     _lib = L
     return L
@@ -372,13 +375,18 @@ class Context:
         _check(fn(self._h, C.byref(xa), C.byref(ya), C.byref(za), count,
                   _tptr(overflow) if overflow is not None else None, _stream_ptr(stream)), f"grns_mpint_{op}")
 
-    def stream_tiles(self, on=True):
-        """LDS-staged tiles (on) or direct loads (off, the default) in eval (test hook)."""
-        _check(lib().grns_debug_stream_tiles(self._h, int(on)), "grns_debug_stream_tiles")
+    def stream_tiles(self, mode=1):
+        """How eval's streaming kernel loads residues (test hook): 0 direct (the default), 1 LDS
+        tiles, 2 next number prefetched into registers."""
+        _check(lib().grns_debug_stream_tiles(self._h, int(mode)), "grns_debug_stream_tiles")
 
     def grid_cap(self, blocks_per_cu=0):
         """Cap the streaming kernels' grids at blocks_per_cu blocks per CU, 0 for none (test hook)."""
         _check(lib().grns_debug_grid_cap(self._h, int(blocks_per_cu)), "grns_debug_grid_cap")
+
+    def rare_staging(self, on=True):
+        """Stage queued numbers' residues for the rare pass (on, the default) or not (test hook)."""
+        _check(lib().grns_debug_rare_staging(self._h, int(on)), "grns_debug_rare_staging")
 
     def reference_loops(self, on=True):
         """Division and scaling through the reference's loops for every number (test hook)."""
@@ -487,6 +495,11 @@ class Batch:
 
     def launch(self, stream=None):
         _check(lib().grns_batch_launch(self._h, _stream_ptr(stream)), "grns_batch_launch")
+
+    def run_per_call(self, stream=None):
+        """Every task through its per-call entry point, one call per task, looped in C++ (the
+        baseline the batched launch replaces; test and benchmark hook)."""
+        _check(lib().grns_debug_batch_per_call(self._h, _stream_ptr(stream)), "grns_debug_batch_per_call")
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -13,6 +13,7 @@
 #include <new>
 #include <vector>
 
+#include "../../include/grns_debug.h"
 #include "grns_internal.hpp"
 
 using namespace grns;
@@ -473,6 +474,41 @@ grns_status grns_batch_set_config(grns_batch *b, const grns_batch_config *cfg) {
 grns_status grns_batch_get_config(const grns_batch *b, grns_batch_config *cfg) {
     if (!b || !cfg) return GRNS_ERR_ARG;
     *cfg = b->cfg;
+    return GRNS_OK;
+}
+
+grns_status grns_debug_batch_per_call(grns_batch *b, void *stream) {
+    if (!b) return GRNS_ERR_ARG;
+    grns_ctx *c = b->ctx;
+    for (const grns_batch_task &u : b->user) {
+        if (u.count == 0) continue;
+        grns_status st = GRNS_OK;
+        switch (u.op) {
+            case GRNS_OP_EVAL:
+            case GRNS_OP_EVAL_FAST:
+                st = grns_eval(c, u.a, u.a_mod_stride, u.a_num_stride, u.count, static_cast<grns_interval *>(u.out0),
+                               u.op == GRNS_OP_EVAL_FAST, stream);
+                break;
+            case GRNS_OP_CMP:
+                st = grns_cmp(c, u.a, u.a_mod_stride, u.a_num_stride, u.b, u.b_mod_stride, u.b_num_stride, u.count,
+                              static_cast<int *>(u.out0), stream);
+                break;
+            case GRNS_OP_MPINT_ADD: st = grns_mpint_add(c, &u.mx, &u.my, &u.mz, u.count, u.overflow, stream); break;
+            case GRNS_OP_MPINT_SUB: st = grns_mpint_sub(c, &u.mx, &u.my, &u.mz, u.count, u.overflow, stream); break;
+            case GRNS_OP_MPINT_MUL: st = grns_mpint_mul(c, &u.mx, &u.my, &u.mz, u.count, u.overflow, stream); break;
+            case GRNS_OP_DIV:
+            case GRNS_OP_DIV_FAST:
+                st = grns_div(c, u.a, u.b, u.a_mod_stride, u.a_num_stride, u.count, static_cast<int *>(u.out0),
+                              static_cast<int *>(u.out1), u.op == GRNS_OP_DIV_FAST ? GRNS_DIV_FAST : 0, stream);
+                break;
+            case GRNS_OP_SCALE2POW:
+                st = grns_scale2pow(c, u.a, u.a_mod_stride, u.a_num_stride, u.D, u.count, static_cast<int *>(u.out0),
+                                    stream);
+                break;
+            default: st = GRNS_ERR_ARG; break;
+        }
+        if (st != GRNS_OK) return st;
+    }
     return GRNS_OK;
 }
 

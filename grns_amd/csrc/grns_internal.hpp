@@ -24,6 +24,7 @@ struct grns_ctx {
         void *stream;
         long long *items;
         unsigned *ctr;
+        int *res;  // staged residues of the first queued items
     };
     std::mutex qmu;
     std::vector<RareQueue> queues;
@@ -38,8 +39,9 @@ struct grns_ctx {
         long long launches = 0;
     };
     bool timing = false;
-    bool stream_tiles = false;      // eval: LDS-staged tiles where the layout allows (test hook, off)
+    int stream_mode = 0;            // eval: 0 direct loads, 1 LDS-staged tiles, 2 next number prefetched (test hook)
     bool reference_loops = false;  // division and scaling: the reference loops for every number (test hook)
+    bool rare_staging = true;      // rare queues stage the residues of queued numbers (test hook)
     int cus = 0;                   // compute units of the device
     int grid_cap = 0;              // streaming kernels: at most grid_cap x cus blocks (0: one per BLOCK numbers)
     StageTimer timers[3];

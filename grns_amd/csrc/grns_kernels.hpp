@@ -173,13 +173,20 @@ GRNS_D void load_number(const int *__restrict__ x, long long smod, long long snu
 // spreads them over every wave group of the device, so that a few long evaluations (refinement
 // loops, mixed-radix conversions) neither stall the streaming waves nor run one after another in
 // the wave that met them.  ctr[0] counts pushes and ctr[2] is set when the queue was full; then the
-// overflowing items carry a marker in their output and the rare pass scans for it.  A one-wave
-// kernel after the rare pass clears the counters for the next call on the stream.  Evaluation,
-// comparison, division and scaling share the queue of a (context, stream) pair, one call at a time.
+// overflowing items carry a marker in their output and the rare pass scans for it.  The
+// thread-per-number kernels also copy a queued number's residues (a pair's, for the comparison)
+// from registers into `res`, RARE_RES_WIDTH ints per slot for the first res_cap slots, so that the
+// rare pass reads 4n contiguous bytes per item instead of one cache line per residue row.  The
+// last block of the rare pass to finish clears the counters for the next call on the stream
+// (ctr[3] counts finished blocks).  Evaluation, comparison, division and scaling share the queue of
+// a (context, stream) pair, one call at a time.
+constexpr int RARE_RES_WIDTH = 128;  // ints per staged slot: two numbers of up to 64 residues
 struct RareQ {
     long long *items;
     unsigned *ctr;
     unsigned cap;
+    int *res;
+    unsigned res_cap;
 };
 
 // Appends j for the lanes with `rare` (all 64 lanes of the wave call it, one atomic per wave).
@@ -202,10 +209,57 @@ GRNS_D bool rare_push(const RareQ &q, bool rare, long long j) {
     return false;
 }
 
-// Clears the counters of a rare-item queue for the next call on its stream (one wave, launched
-// after the rare pass).
+// rare_push that also stages the residues of the queued numbers: xr (and yr, for a pair: PAIR) of
+// the lanes whose slot is below res_cap, at res[slot * RARE_RES_WIDTH] (x) and + N (y).  The
+// arrays stay register arrays (references, indices known at compile time).
+template <int N, bool PAIR>
+GRNS_D bool rare_push_staged(const RareQ &q, bool rare, long long j, const int (&xr)[N], const int (&yr)[N]) {
+    const unsigned long long m = __ballot(rare);
+    if (!m) return true;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(&q.ctr[0], (unsigned)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (!rare) return true;
+    const unsigned slot = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+    if (slot < q.cap) {
+        if (slot < q.res_cap) {
+            int *r = q.res + (size_t)slot * RARE_RES_WIDTH;
+#pragma unroll
+            for (int i = 0; i < N; i++) r[i] = xr[i];
+            if constexpr (PAIR) {
+#pragma unroll
+                for (int i = 0; i < N; i++) r[N + i] = yr[i];
+            }
+        }
+        q.items[slot] = j;
+        return true;
+    }
+    q.ctr[2] = 1u;
+    return false;
+}
+
+// Clears the counters of a rare-item queue (one wave; launched once when the queue is made).
 static __global__ void k_rare_reset(unsigned *ctr) {
     if (threadIdx.x < 4) ctr[threadIdx.x] = 0u;
+}
+
+// End of a rare pass, reached by every thread of every block: the last block to get here clears
+// the queue's counters for the next call on the stream.  Every block has read the counters at its
+// start, before it counts itself finished, so none reads them after the clear.
+GRNS_D void rare_finish(const RareQ &q) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned done = atomicAdd(&q.ctr[3], 1u);
+        if (done == gridDim.x - 1) {
+            q.ctr[0] = 0u;
+            q.ctr[1] = 0u;
+            q.ctr[2] = 0u;
+            q.ctr[3] = 0u;
+        }
+    }
 }
 
 // Marker of an evaluation left to the rare pass after a queue overflow (no result has it).
@@ -253,22 +307,23 @@ GRNS_D void eval_step(const DevCtx *__restrict__ c, const int *__restrict__ x, l
 }
 
 // Plain per-call evaluation, streaming pass: one number per lane; numbers that need the
-// cooperative algorithm go to the rare-item queue.
+// cooperative algorithm go to the rare-item queue with their residues.  Up to n = 32 the kernel
+// keeps 4 waves per SIMD (at most 128 VGPRs), as it has without the staged copy.
 template <int N, bool FAST, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_eval(const DevCtx *__restrict__ c, const int *__restrict__ x, long long smod,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(N <= 32 ? 4 : 1))) void k_eval(const DevCtx *__restrict__ c, const int *__restrict__ x, long long smod,
                                                 long long snum, long long count, interval_t *__restrict__ out,
                                                 RareQ q) {
     for (long long base = (long long)blockIdx.x * BLOCK; base < count; base += (long long)gridDim.x * BLOCK) {
         const long long j = base + threadIdx.x;
         bool rare = false;
+        int xr[N];
         if (j < count) {
-            int xr[N];
             load_number<N, MODE>(x, smod, snum, j, xr);
             interval_t r;
             if (eval_hot<N, FAST>(c, xr, r)) store_interval(out + j, r);
             else rare = true;
         }
-        if (!rare_push(q, rare, j)) {
+        if (!rare_push_staged<N, false>(q, rare, j, xr, xr)) {
             interval_t mk;
             mk.low.frac = __builtin_nan(""); mk.low.exp = EVAL_MARK;
             mk.upp.frac = __builtin_nan(""); mk.upp.exp = EVAL_MARK;
@@ -319,7 +374,7 @@ GRNS_D void eval_one(const DevCtx *__restrict__ c, const int (&xr)[N], long long
     bool rare = false;
     if (eval_hot<N, FAST>(c, xr, r)) store_interval(out + j, r);
     else rare = true;
-    if (!rare_push(q, rare, j)) {
+    if (!rare_push_staged<N, false>(q, rare, j, xr, xr)) {
         interval_t mk;
         mk.low.frac = __builtin_nan(""); mk.low.exp = EVAL_MARK;
         mk.upp.frac = __builtin_nan(""); mk.upp.exp = EVAL_MARK;
@@ -361,6 +416,31 @@ __global__ __launch_bounds__(BLOCK) void k_eval_tiles(const DevCtx *__restrict__
             load_number<N, LD_MM>(x, smod, 1, j, xr);
             eval_one<N, FAST>(c, xr, j, out, q);
         }
+    }
+}
+
+// rns_eval_compute[_fast] with the residues of the lane's next number loaded while the current one
+// is evaluated (two register sets, named separately and the loop unrolled by two): a persistent
+// grid in which every wave keeps one number's loads in flight at all times.  Modulus-major rows,
+// count <= 2^29, n <= 32 (checked by the launcher; at least 3 waves per SIMD are asked for).
+// The rare queue and its pass as k_eval.
+template <int N, bool FAST>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_eval_pf(const DevCtx *__restrict__ c, const int *__restrict__ x,
+                                                   long long smod, long long count, interval_t *__restrict__ out,
+                                                   RareQ q) {
+    const long long stride = (long long)gridDim.x * BLOCK;
+    long long j = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    const long long j0 = j - threadIdx.x;  // the block's first number: uniform loop bound
+    int xa[N], xb[N];
+    if (j < count) load_number<N, LD_MM>(x, smod, 1, j, xa);
+    for (long long b = j0; b < count; b += 2 * stride) {
+        const long long ja = b + threadIdx.x, jb = ja + stride;
+        if (jb < count) load_number<N, LD_MM>(x, smod, 1, jb, xb);
+        if (ja < count) eval_one<N, FAST>(c, xa, ja, out, q);
+        if (b + stride >= count) break;
+        const long long jc = jb + stride;
+        if (jc < count) load_number<N, LD_MM>(x, smod, 1, jc, xa);
+        if (jb < count) eval_one<N, FAST>(c, xb, jb, out, q);
     }
 }
 
@@ -668,20 +748,20 @@ GRNS_D void cmp_step(const DevCtx *__restrict__ c, const int *__restrict__ x, lo
 // Plain per-call comparison, streaming pass: one pair per lane, no cooperative code (so the loop
 // keeps its registers); pairs the lane cannot settle go to the rare-item queue.
 template <int N, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_cmp(const DevCtx *__restrict__ c, const int *__restrict__ x, long long xsm,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(N <= 32 ? 5 : 1))) void k_cmp(const DevCtx *__restrict__ c, const int *__restrict__ x, long long xsm,
                                                long long xsn, const int *__restrict__ y, long long ysm, long long ysn,
                                                long long count, int *__restrict__ sign, RareQ q) {
     for (long long base = (long long)blockIdx.x * BLOCK; base < count; base += (long long)gridDim.x * BLOCK) {
         const long long j = base + threadIdx.x;
         int res = 0;
+        int xr[N], yr[N];
         if (j < count) {
-            int xr[N], yr[N];
             load_number<N, MODE>(x, xsm, xsn, j, xr);
             load_number<N, MODE>(y, ysm, ysn, j, yr);
             res = cmp_lane<N, false>(c, xr, yr, x + j * xsn, xsm, y + j * ysn, ysm);
             if (res != 2) sign[j] = res;
         }
-        if (!rare_push(q, res == 2, j)) sign[j] = 2;  // marker: left to the scan of the rare pass
+        if (!rare_push_staged<N, true>(q, res == 2, j, xr, yr)) sign[j] = 2;  // marker: left to the scan of the rare pass
     }
 }
 
@@ -891,11 +971,13 @@ __global__ __launch_bounds__(BLOCK) void k_eval_rare(const DevCtx *__restrict__ 
     for (long long k = gid; k < total; k += ngroups) {
         const long long j = rare_item(q, nq, k, total);
         if (k >= nq && reinterpret_cast<const long long *>(out + j)[1] != EVAL_MARK) continue;
-        const int xi = g.gl < N ? x[j * snum + (long long)g.gl * smod] : 0;
+        const int *src = k < (long long)q.res_cap && k < nq ? q.res + k * RARE_RES_WIDTH : nullptr;
+        const int xi = g.gl < N ? (src ? src[g.gl] : x[j * snum + (long long)g.gl * smod]) : 0;
         interval_t r = group_eval<FAST>(c, g, xi, st);
         if (g.gl == 0) store_interval(out + j, r);
     }
     if (g.gl == 0 && st) atomicOr(status, st);
+    rare_finish(q);
 }
 
 // Rare pass of the comparison: one wave group per item.
@@ -916,12 +998,14 @@ __global__ __launch_bounds__(BLOCK) void k_cmp_rare(const DevCtx *__restrict__ c
     for (long long k = gid; k < total; k += ngroups) {
         const long long j = rare_item(q, nq, k, total);
         if (k >= nq && sign[j] != 2) continue;
-        const int xi = g.gl < N ? x[j * xsn + (long long)g.gl * xsm] : 0;
-        const int yi = g.gl < N ? y[j * ysn + (long long)g.gl * ysm] : 0;
+        const int *src = k < (long long)q.res_cap && k < nq ? q.res + k * RARE_RES_WIDTH : nullptr;
+        const int xi = g.gl < N ? (src ? src[g.gl] : x[j * xsn + (long long)g.gl * xsm]) : 0;
+        const int yi = g.gl < N ? (src ? src[N + g.gl] : y[j * ysn + (long long)g.gl * ysm]) : 0;
         const int r = group_cmp(c, g, xi, yi, st);
         if (g.gl == 0) sign[j] = r;
     }
     if (g.gl == 0 && st) atomicOr(status, st);
+    rare_finish(q);
 }
 
 // ---------------------------------------------------------------- arg-max (rns_max)
@@ -1638,6 +1722,7 @@ __global__ __launch_bounds__(BLOCK) void k_div_rare(const DevCtx *__restrict__ c
         }
     }
     if (g.gl == 0 && st) atomicOr(status, st);
+    rare_finish(rq);
 }
 
 // The reference loop for every pair.
@@ -1935,6 +2020,7 @@ static __global__ __launch_bounds__(BLOCK) void k_scale2pow_rare(const DevCtx *_
         const int v = group_scale2pow(c, g, xi, D[j]);
         if (g.gl < g.L) y[off] = v;
     }
+    rare_finish(rq);
 }
 
 static __global__ __launch_bounds__(BLOCK) void k_scale2pow(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,

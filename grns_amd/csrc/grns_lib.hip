@@ -210,7 +210,7 @@ hipStream_t S(void *s) { return (hipStream_t)s; }
 // The LDS-staged streaming kernels: for n = 16 and 32 (two tiles of n x BLOCK residues fit the
 // LDS of a CU twice over), modulus-major rows aligned to 16 bytes, at least one full tile.
 bool tiles_ok(const grns_ctx *c, int n, const int *x, long long smod, size_t count) {
-    return c->stream_tiles && (n == 16 || n == 32) && ((uintptr_t)x & 15) == 0 && (smod & 3) == 0 &&
+    return c->stream_mode == 1 && (n == 16 || n == 32) && ((uintptr_t)x & 15) == 0 && (smod & 3) == 0 &&
            count >= (size_t)BLOCK;
 }
 
@@ -224,6 +224,17 @@ long long tiles_grid(const grns_ctx *c, size_t count, const void *kernel) {
     long long tiles = (long long)(count / BLOCK);
     long long g = (long long)cus * per;
     return g < tiles ? g : (tiles > 0 ? tiles : 1);
+}
+
+// Grid of the prefetching streaming kernels: the blocks resident at once (occupancy x CUs, or the
+// context's cap per CU when lower), or fewer when there are fewer blocks of numbers.
+long long pf_grid(const grns_ctx *c, size_t count, const void *kernel) {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, BLOCK, 0) != hipSuccess || per < 1) per = 1;
+    if (c->grid_cap > 0 && c->grid_cap < per) per = c->grid_cap;
+    long long blocks = (long long)((count + BLOCK - 1) / BLOCK);
+    long long g = (long long)c->cus * per;
+    return g < blocks ? g : (blocks > 0 ? blocks : 1);
 }
 
 // Grid of the thread-per-number kernels of the large moduli sets (n > 64): blocks of one wave,
@@ -273,10 +284,12 @@ struct DeviceGuard {
 };
 
 constexpr unsigned RARE_CAP = 1u << 20;  // queued rare items per stream (8 MiB of indices)
+constexpr unsigned RARE_RES_CAP = 1u << 16;  // of which staged with their residues (32 MiB)
 
 // The rare-item queue of `stream` (allocated and cleared on the first call on that stream).  The
-// counters are cleared by a kernel on `stream` itself: a hipMemset is ordered on the null stream
-// only, and the kernels of a non-blocking stream could otherwise read the counters before it.
+// counters are first cleared by a kernel on `stream` itself (a hipMemset is ordered on the null
+// stream only, and the kernels of a non-blocking stream could otherwise read the counters before
+// it); after that, each rare pass clears them when its last block finishes.
 grns_status rare_queue(grns_ctx *c, void *stream, RareQ &q) {
     std::lock_guard<std::mutex> lock(c->qmu);
     for (auto &e : c->queues)
@@ -284,24 +297,31 @@ grns_status rare_queue(grns_ctx *c, void *stream, RareQ &q) {
             q.items = e.items;
             q.ctr = e.ctr;
             q.cap = RARE_CAP;
+            q.res = e.res;
+            q.res_cap = c->rare_staging ? RARE_RES_CAP : 0u;
             return GRNS_OK;
         }
-    grns_ctx::RareQueue e{stream, nullptr, nullptr};
+    grns_ctx::RareQueue e{stream, nullptr, nullptr, nullptr};
     if (hipMalloc(&e.items, sizeof(long long) * RARE_CAP) != hipSuccess) return GRNS_ERR_OOM;
-    if (hipMalloc(&e.ctr, 4 * sizeof(unsigned)) != hipSuccess) {
+    if (hipMalloc(&e.ctr, 4 * sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&e.res, sizeof(int) * RARE_RES_WIDTH * (size_t)RARE_RES_CAP) != hipSuccess) {
         (void)hipFree(e.items);
+        if (e.ctr) (void)hipFree(e.ctr);
         return GRNS_ERR_OOM;
     }
     hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), e.ctr);
     if (hipGetLastError() != hipSuccess) {
         (void)hipFree(e.items);
         (void)hipFree(e.ctr);
+        (void)hipFree(e.res);
         return GRNS_ERR_HIP;
     }
     c->queues.push_back(e);
     q.items = e.items;
     q.ctr = e.ctr;
     q.cap = RARE_CAP;
+    q.res = e.res;
+    q.res_cap = c->rare_staging ? RARE_RES_CAP : 0u;
     return GRNS_OK;
 }
 
@@ -462,6 +482,7 @@ grns_status grns_ctx_destroy(grns_ctx *c) {
         for (auto &q : c->queues) {
             (void)hipFree(q.items);
             (void)hipFree(q.ctr);
+            (void)hipFree(q.res);
         }
         for (auto &t : c->timers)
             for (void *e : t.ev) (void)hipEventDestroy((hipEvent_t)e);
@@ -534,6 +555,11 @@ grns_status grns_eval(grns_ctx *c, const int *x, long long smod, long long snum,
                                                         : (const void *)k_eval_tiles<NN, false>));                \
             if (fast) hipLaunchKernelGGL((k_eval_tiles<NN, true>), tg, blk, 0, S(stream), c->d, x, smod, cnt, o, q); \
             else hipLaunchKernelGGL((k_eval_tiles<NN, false>), tg, blk, 0, S(stream), c->d, x, smod, cnt, o, q);  \
+        } else if (mm && c->stream_mode == 2 && NN <= 32) {                                                      \
+            dim3 pg((unsigned)pf_grid(c, count, fast ? (const void *)k_eval_pf<NN, true>                         \
+                                                     : (const void *)k_eval_pf<NN, false>));                      \
+            if (fast) hipLaunchKernelGGL((k_eval_pf<NN, true>), pg, blk, 0, S(stream), c->d, x, smod, cnt, o, q);  \
+            else hipLaunchKernelGGL((k_eval_pf<NN, false>), pg, blk, 0, S(stream), c->d, x, smod, cnt, o, q);     \
         } else if (mm) {                                                                                         \
             if (fast) hipLaunchKernelGGL((k_eval<NN, true, LD_MM>), grid, blk, 0, S(stream), c->d, x, smod, snum,  \
                                          cnt, o, q);                                                             \
@@ -552,7 +578,6 @@ grns_status grns_eval(grns_ctx *c, const int *x, long long smod, long long snum,
         else hipLaunchKernelGGL((k_eval_rare<NN, false>), rg, blk, 0, S(stream), c->d, x, smod, snum, cnt, o,      \
                                 c->d_status, q);                                                                 \
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                                \
-        hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), q.ctr);                                \
         return launch_check();                                                                                   \
     }
     switch (c->n) {
@@ -605,7 +630,6 @@ grns_status grns_cmp(grns_ctx *c, const int *x, long long xsm, long long xsn, co
         hipLaunchKernelGGL((k_cmp_rare<NN>), rg, blk, 0, S(stream), c->d, x, xsm, xsn, y, ysm, ysn, cnt, sign,       \
                            c->d_status, q);                                                                        \
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                                  \
-        hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), q.ctr);                                  \
         return launch_check();                                                                                     \
     }
     switch (c->n) {
@@ -668,7 +692,6 @@ grns_status grns_div(grns_ctx *c, const int *x, const int *d, long long smod, lo
         if (fast) hipLaunchKernelGGL((k_div_rare<true>), rg, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, c->d_status, rq);
         else hipLaunchKernelGGL((k_div_rare<false>), rg, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, c->d_status, rq);
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;
-        hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), rq.ctr);
         return launch_check();
     }
     if (fast) hipLaunchKernelGGL((k_div<true>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, c->d_status);
@@ -784,7 +807,6 @@ grns_status grns_scale2pow(grns_ctx *c, const int *x, long long smod, long long 
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;
         hipLaunchKernelGGL(k_scale2pow_rare, rg, blk, 0, S(stream), c->d, G, x, smod, snum, D, cnt, y, rq);
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;
-        hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), rq.ctr);
         return launch_check();
     }
     hipLaunchKernelGGL(k_scale2pow, grid, blk, 0, S(stream), c->d, G, x, smod, snum, D, cnt, y);
@@ -1271,15 +1293,21 @@ grns_status grns_debug_stage_read(grns_ctx *c, int op, double *ms_total, long lo
     return GRNS_OK;
 }
 
+grns_status grns_debug_rare_staging(grns_ctx *c, int on) {
+    if (!c) return GRNS_ERR_ARG;
+    c->rare_staging = on != 0;
+    return GRNS_OK;
+}
+
 grns_status grns_debug_grid_cap(grns_ctx *c, int blocks_per_cu) {
     if (!c || blocks_per_cu < 0) return GRNS_ERR_ARG;
     c->grid_cap = blocks_per_cu;
     return GRNS_OK;
 }
 
-grns_status grns_debug_stream_tiles(grns_ctx *c, int on) {
-    if (!c) return GRNS_ERR_ARG;
-    c->stream_tiles = on != 0;
+grns_status grns_debug_stream_tiles(grns_ctx *c, int mode) {
+    if (!c || mode < 0 || mode > 2) return GRNS_ERR_ARG;
+    c->stream_mode = mode;
     return GRNS_OK;
 }
 

@@ -34,15 +34,25 @@ grns_status grns_debug_stage_read(grns_ctx *ctx, int op, double *ms_total, long 
  * the reference's loops (src/rnsdiv.cuh, src/rnsscal.cuh) for every number instead of the word
  * forms, so that the two can be compared on the device. */
 grns_status grns_debug_reference_loops(grns_ctx *ctx, int on);
-/* Test hook: on = 1 makes grns_eval (n = 16 and 32, modulus-major rows aligned to 16 bytes) stage
- * its residues through double-buffered LDS tiles instead of loading them straight into registers,
- * the default.  Both forms give the same results; the tiled one is slower on gfx950 (0.58 against
- * 0.49 ms for 2^24 numbers at n = 32: its 185 VGPRs halve the waves per SIMD). */
-grns_status grns_debug_stream_tiles(grns_ctx *ctx, int on);
+/* Test hook: how the thread-per-number kernel of grns_eval gets its residues (modulus-major rows,
+ * at most 2^29 numbers).  mode 0 (the default): straight into registers, one block per 256
+ * numbers.  mode 1 (n = 16 and 32, rows aligned to 16 bytes): through double-buffered LDS tiles.
+ * mode 2: a persistent grid whose lanes load their next number while evaluating the current one.
+ * All modes give the same results; the hook lets them be compared and timed on the device. */
+grns_status grns_debug_stream_tiles(grns_ctx *ctx, int mode);
 /* Test hook: caps the grid of the per-number streaming kernels of grns_eval, grns_cmp and the mpint
  * operations (grid-stride loops) at blocks_per_cu blocks per compute unit; 0 restores one block per
  * 256 numbers.  Results do not depend on it; the hook lets grid shapes be timed on the device. */
 grns_status grns_debug_grid_cap(grns_ctx *ctx, int blocks_per_cu);
+/* Test hook: on = 0 stops the streaming kernels of grns_eval and grns_cmp from copying the residues
+ * of the numbers they queue for the rare pass (which then reads them from the inputs); 1, the
+ * default, restores it.  Results do not depend on it. */
+grns_status grns_debug_rare_staging(grns_ctx *ctx, int on);
+/* Baseline of the batched form: runs every task of the batch through its per-call entry point
+ * (grns_eval, grns_cmp, grns_mpint_*, grns_div, grns_scale2pow), one call per task in table order
+ * on `stream`, from C++ -- what a caller without the batched form does.  Results equal those of
+ * grns_batch_launch. */
+grns_status grns_debug_batch_per_call(grns_batch *b, void *stream);
 /* Scheduler test hook: a server whose launches run nothing and complete delay_us after issue;
  * it accepts host-only contexts (device -1), so admission and batching run without a GPU. */
 grns_status grns_serve_create_sim(const grns_serve_config *cfg, int delay_us, grns_server **out);

@@ -190,3 +190,22 @@ def test_batch_tuner_keeps_results(dev):
     torch.cuda.synchronize()
     assert _got(jobs) == ref
     b.close()
+
+
+@pytest.mark.parametrize("n", [8, 64])
+def test_batch_per_call_loop_matches(dev, n):
+    """The C++ per-call loop over a batch's tasks (the batched form's baseline) gives the per-call
+    and oracle results too."""
+    import grns_amd as G
+    ctx, o, jobs = _build(n, seed=4000 + n)
+    ref = _plain(ctx, jobs)
+    b = G.Batch(ctx, _tasks(jobs))
+    _zero(jobs)
+    b.run_per_call()
+    torch.cuda.synchronize()
+    got = _got(jobs)
+    assert got == ref
+    for (kind, _, _, want), g in zip(jobs, got):
+        assert g == want, kind
+    assert ctx.status() == 0
+    b.close()

@@ -57,16 +57,18 @@ def test_eval_parity(ctxs, dev, n, fast):
 
 @pytest.mark.parametrize("n", [16, 32])
 @pytest.mark.parametrize("fast", [False, True])
-def test_eval_lds_tiles_form(dev, n, fast):
-    """The LDS-staged eval (a test hook, off by default; taken for 16-byte aligned rows whose
-    stride is a multiple of 4) equals the oracle: whole tiles and a ragged last one, with a packed
-    row stride and with rows padded to a wider stride."""
+@pytest.mark.parametrize("mode", [1, 2])
+def test_eval_streaming_forms(dev, n, fast, mode):
+    """The other streaming forms of eval (test hooks, off by default) equal the oracle: LDS-staged
+    tiles (mode 1, taken for 16-byte aligned rows whose stride is a multiple of 4) and the
+    persistent grid that prefetches each lane's next number (mode 2).  Whole blocks and a ragged
+    last one, with a packed row stride and with rows padded to a wider stride."""
     import grns_amd as G
     from oracle.oracle import Oracle
     mod = G.largest_primes_below(n)
     c, o = G.Context(mod, 0), Oracle(mod)
     try:
-        c.stream_tiles(True)
+        c.stream_tiles(mode)
         vals = eval_inputs(mod, 5 * 256 + 76, seed=40 + n)
         x = to_res(vals, mod)
         got = c.eval(_dev(x, dev), fast=fast)
@@ -541,11 +543,14 @@ def test_rare_queue_overflow(ctxs, dev, n):
 
 
 @pytest.mark.parametrize("n", [8, 32])
-def test_rare_items_distinct(ctxs, dev, n):
+@pytest.mark.parametrize("staging", [True, False])
+def test_rare_items_distinct(ctxs, dev, n, staging):
     """2^17 + 999 distinct numbers just below M in shuffled order (each needs the rare pass of
-    evaluation and of comparison): every queued item is resolved for its own number; every
-    interval (as bytes) and sign equals the CPU path's."""
+    evaluation and of comparison): every queued item is resolved for its own number (the first
+    2^16 from residues staged by the streaming pass, the rest, or all of them with staging off,
+    from the inputs); every interval (as bytes) and sign equals the CPU path's."""
     c, o, mod = ctxs[n]
+    c.rare_staging(staging)
     from grns_amd.moduli import product
     M = product(mod)
     count = (1 << 17) + 999
@@ -558,6 +563,7 @@ def test_rare_items_distinct(ctxs, dev, n):
     xd, yd = _dev(xa, dev), _dev(ya, dev)
     assert _intervals_bytes(c.eval(xd)) == o.eval_array(xa).tobytes()
     assert (c.cmp(xd, yd).cpu().numpy() == o.cmp_array(xa, ya)).all()
+    c.rare_staging(True)
     assert c.status() == 0
 
 
