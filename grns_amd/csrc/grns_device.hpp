@@ -107,6 +107,12 @@ struct interval_t {
 };
 
 constexpr int MAXN = 256;      // largest moduli set a context accepts
+constexpr int WORDS_BATCH = 8;  // table rows whose loads the word conversions issue together
+
+// Pointers into global memory (address space 1): loads through them are global loads (a scalar
+// base and a vector offset), not flat loads, which also wait on the LDS counter.
+typedef const __attribute__((address_space(1))) int global_int;
+typedef const __attribute__((address_space(1))) unsigned global_uint;
 constexpr int THRESH = 30;     // RNS_P2_SCALING_THRESHOLD, src/params.h:22
 constexpr int KEY_SHIFT = 26;  // arg-max keys of values near 0 or M: the key of 2^KEY_SHIFT x
 
@@ -769,13 +775,22 @@ GRNS_D unsigned group_words(const DevCtx *c, const Group &g, int xi) {
     const double fr = sum - kf;
     const int WW = c->words;
     const bool wl = g.gl < WW;
-    const unsigned *tw = c->crt_words + g.gl;
+    // word gl of M_i for i = 0 .. L-1: the loads of 8 rows are issued together (global loads from
+    // an address every lane may read), so that the table's latency is paid once per 8 terms
+    const global_uint *tw = (const global_uint *)(uintptr_t)c->crt_words + (wl ? g.gl : 0);
     unsigned long long lo = 0, hi = 0;
-    for (int i = 0; i < g.L; i++) {
-        const unsigned si = gword<GD>(g, (unsigned)s, i);
-        const unsigned long long p = (unsigned long long)si * (wl ? tw[(size_t)i * WW] : 0u);
-        lo += (unsigned)p;
-        hi += p >> 32;
+    for (int i0 = 0; i0 < g.L; i0 += WORDS_BATCH) {
+        unsigned t[WORDS_BATCH];
+#pragma unroll
+        for (int k = 0; k < WORDS_BATCH; k++) t[k] = i0 + k < g.L ? tw[(size_t)(i0 + k) * WW] : 0u;
+#pragma unroll
+        for (int k = 0; k < WORDS_BATCH; k++) {
+            if (i0 + k >= g.L) break;  // the same on every lane
+            const unsigned si = gword<GD>(g, (unsigned)s, i0 + k);
+            const unsigned long long p = (unsigned long long)si * (wl ? t[k] : 0u);
+            lo += (unsigned)p;
+            hi += p >> 32;
+        }
     }
     // words of sum_i s_i M_i: lo_j + hi_{j-1} (< 2^39), then the carries
     const unsigned hlo = wlower<GD>(g, (unsigned)hi), hhi = wlower<GD>(g, (unsigned)(hi >> 32));
@@ -804,14 +819,22 @@ template <int GD>
 GRNS_D int group_words_residue(const DevCtx *c, const Group &g, unsigned v, int nw) {
     const bool act = g.gl < g.L;
     const int n = c->n;
-    const int *p2 = c->pow2 + (act ? g.gl : 0);
+    // 2^(32 k) mod m_gl for 8 words at a time (loads issued together, as in group_words)
+    const global_int *p2 = (const global_int *)(uintptr_t)c->pow2 + (act ? g.gl : 0);
     unsigned long long lo = 0, hi = 0;
     if (nw > (c->log2M >> 5) + 1) nw = (c->log2M >> 5) + 1;  // a value below M has no more words
-    for (int k = 0; k < nw; k++) {
-        const unsigned wk = gword<GD>(g, v, k);
-        const unsigned long long p = (unsigned long long)wk * (unsigned)(act ? p2[(size_t)(32 * k) * n] : 0);
-        lo += (unsigned)p;
-        hi += p >> 32;
+    for (int k0 = 0; k0 < nw; k0 += WORDS_BATCH) {
+        int t[WORDS_BATCH];
+#pragma unroll
+        for (int k = 0; k < WORDS_BATCH; k++) t[k] = k0 + k < nw ? p2[(size_t)(32 * (k0 + k)) * n] : 0;
+#pragma unroll
+        for (int k = 0; k < WORDS_BATCH; k++) {
+            if (k0 + k >= nw) break;  // the same on every lane
+            const unsigned wk = gword<GD>(g, v, k0 + k);
+            const unsigned long long p = (unsigned long long)wk * (unsigned)(act ? t[k] : 0);
+            lo += (unsigned)p;
+            hi += p >> 32;
+        }
     }
     if (!act) return 0;
     const int m = c->m[g.gl];

@@ -117,9 +117,6 @@ GRNS_D void store_interval(interval_t *p, const interval_t &v) {
     q[1] = make_double2(v.upp.frac, __longlong_as_double(v.upp.exp));
 }
 
-// A pointer into global memory (address space 1): loads through it are global loads, which take a
-// scalar base and a 32-bit vector offset.
-typedef const __attribute__((address_space(1))) int global_int;
 
 // Residues of number j into registers.  LD_MM: modulus-major (snum == 1) and j < 2^29, checked by
 // the launcher: a wave-uniform row pointer stepped from modulus to modulus plus one 32-bit byte
