@@ -761,37 +761,18 @@ GRNS_D void wmul_sub(const Group &g, unsigned &u, unsigned v, unsigned q) {
     u = w2 - (unsigned)((bin >> g.gl) & 1);
 }
 
-// The value of canonical residues (residue xi on lane gl < n) as words, by the Chinese remainder
-// theorem: x = sum_i s_i (M / m_i) - k M with s_i = x_i w_i mod m_i and k = floor(sum_i s_i / m_i).
-// k comes from a double sum and is checked against the words when that sum is near an integer.
+// The value of canonical residues as words, by the Chinese remainder theorem:
+// x = sum_i s_i (M / m_i) - k M with s_i = x_i w_i mod m_i and k = floor(sum_i s_i / m_i).  k comes
+// from a double sum and is checked against the words when that sum is near an integer.
+// crt_finish: the words of sum_i s_i M_i from the lane's column sums (lo: low halves of the terms
+// of word gl, hi: their high halves, which belong to word gl + 1), the carries, and k M removed.
 template <int GD>
-GRNS_D unsigned group_words(const DevCtx *c, const Group &g, int xi) {
-    const bool act = g.gl < g.L;
-    // exact reduction: the reciprocal one (mulmod_recip) can return s_i - m_i or wrap, and the
-    // unsigned word sums below need 0 <= s_i < m_i
-    const int s = act ? mulmod3(xi, c->w[g.gl], c->m[g.gl], c->dinv[g.gl]) : 0;
-    const double sum = group_sum_d(g, act ? (double)s * c->dinv[g.gl] : 0.0);
+GRNS_D unsigned crt_finish(const DevCtx *c, const Group &g, unsigned long long lo, unsigned long long hi,
+                           double sum) {
     const double kf = floor(sum);
     const double fr = sum - kf;
     const int WW = c->words;
     const bool wl = g.gl < WW;
-    // word gl of M_i for i = 0 .. L-1: the loads of 8 rows are issued together (global loads from
-    // an address every lane may read), so that the table's latency is paid once per 8 terms
-    const global_uint *tw = (const global_uint *)(uintptr_t)c->crt_words + (wl ? g.gl : 0);
-    unsigned long long lo = 0, hi = 0;
-    for (int i0 = 0; i0 < g.L; i0 += WORDS_BATCH) {
-        unsigned t[WORDS_BATCH];
-#pragma unroll
-        for (int k = 0; k < WORDS_BATCH; k++) t[k] = i0 + k < g.L ? tw[(size_t)(i0 + k) * WW] : 0u;
-#pragma unroll
-        for (int k = 0; k < WORDS_BATCH; k++) {
-            if (i0 + k >= g.L) break;  // the same on every lane
-            const unsigned si = gword<GD>(g, (unsigned)s, i0 + k);
-            const unsigned long long p = (unsigned long long)si * (wl ? t[k] : 0u);
-            lo += (unsigned)p;
-            hi += p >> 32;
-        }
-    }
     // words of sum_i s_i M_i: lo_j + hi_{j-1} (< 2^39), then the carries
     const unsigned hlo = wlower<GD>(g, (unsigned)hi), hhi = wlower<GD>(g, (unsigned)(hi >> 32));
     const unsigned long long v = lo + hlo + ((unsigned long long)hhi << 32);
@@ -800,7 +781,7 @@ GRNS_D unsigned group_words(const DevCtx *c, const Group &g, int xi) {
     const unsigned w = (unsigned)t;
     const unsigned long long cin = carry_in_bits(gballot<GD>(g, (t >> 32) != 0), gballot<GD>(g, w == 0xffffffffu));
     unsigned X = w + (unsigned)((cin >> g.gl) & 1);
-    const unsigned mw = wl ? tw[(size_t)g.L * WW] : 0u;
+    const unsigned mw = wl ? c->crt_words[(size_t)g.L * WW + g.gl] : 0u;
     wmul_sub<GD>(g, X, mw, (unsigned)kf);
     if (!(fr > 0x1p-30 && fr < 1.0 - 0x1p-30)) {
         if (wneg<GD>(g, X)) {
@@ -812,6 +793,69 @@ GRNS_D unsigned group_words(const DevCtx *c, const Group &g, int xi) {
         }
     }
     return X;
+}
+
+// The column sums of K numbers at once: s[k] on lane gl < L is the k-th number's s_gl; word gl of
+// M_i is loaded once for all K (8 rows of the table per batch of global loads, so that its latency
+// is paid once per 8 terms).
+template <int GD, int K>
+GRNS_D void crt_columns(const DevCtx *c, const Group &g, const int (&s)[K], unsigned long long (&lo)[K],
+                        unsigned long long (&hi)[K]) {
+    const int WW = c->words;
+    const bool wl = g.gl < WW;
+    const global_uint *tw = (const global_uint *)(uintptr_t)c->crt_words + (wl ? g.gl : 0);
+#pragma unroll
+    for (int k = 0; k < K; k++) lo[k] = hi[k] = 0;
+    for (int i0 = 0; i0 < g.L; i0 += WORDS_BATCH) {
+        unsigned t[WORDS_BATCH];
+#pragma unroll
+        for (int b = 0; b < WORDS_BATCH; b++) t[b] = i0 + b < g.L ? tw[(size_t)(i0 + b) * WW] : 0u;
+#pragma unroll
+        for (int b = 0; b < WORDS_BATCH; b++) {
+            if (i0 + b >= g.L) break;  // the same on every lane
+            const unsigned tb = wl ? t[b] : 0u;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const unsigned long long p = (unsigned long long)gword<GD>(g, (unsigned)s[k], i0 + b) * tb;
+                lo[k] += (unsigned)p;
+                hi[k] += p >> 32;
+            }
+        }
+    }
+}
+
+// Words of the group's number with canonical residue xi (lane gl < n).  The reduction of s is the
+// exact one: the reciprocal one (mulmod_recip) can return s_i - m_i or wrap, and the unsigned word
+// sums need 0 <= s_i < m_i.
+template <int GD>
+GRNS_D unsigned group_words(const DevCtx *c, const Group &g, int xi) {
+    const bool act = g.gl < g.L;
+    const double dinv = act ? c->dinv[g.gl] : 1.0;
+    const int s[1] = {act ? mulmod3(xi, c->w[g.gl], c->m[g.gl], dinv) : 0};
+    const double sum = group_sum_d(g, act ? (double)s[0] * dinv : 0.0);
+    unsigned long long lo[1], hi[1];
+    crt_columns<GD, 1>(c, g, s, lo, hi);
+    return crt_finish<GD>(c, g, lo[0], hi[0], sum);
+}
+
+// The words of two numbers (x and d of a division) in one pass over the table.
+template <int GD>
+GRNS_D void group_words_pair(const DevCtx *c, const Group &g, int xi, int di, unsigned &X, unsigned &D) {
+    const bool act = g.gl < g.L;
+    int m = 1, w = 0;
+    double dinv = 1.0;
+    if (act) {
+        m = c->m[g.gl];
+        w = c->w[g.gl];
+        dinv = c->dinv[g.gl];
+    }
+    const int s[2] = {act ? mulmod3(xi, w, m, dinv) : 0, act ? mulmod3(di, w, m, dinv) : 0};
+    const double sx = group_sum_d(g, act ? (double)s[0] * dinv : 0.0);
+    const double sd = group_sum_d(g, act ? (double)s[1] * dinv : 0.0);
+    unsigned long long lo[2], hi[2];
+    crt_columns<GD, 2>(c, g, s, lo, hi);
+    X = crt_finish<GD>(c, g, lo[0], hi[0], sx);
+    D = crt_finish<GD>(c, g, lo[1], hi[1], sd);
 }
 
 // Residue modulo m_gl (lanes gl < n) of the value held in the first nw words.
@@ -850,8 +894,8 @@ GRNS_D int group_words_residue(const DevCtx *c, const Group &g, unsigned v, int 
 template <int GD>
 GRNS_D void group_div_words(const DevCtx *c, const Group &g, int xi, int di, int &q, int &r) {
     const bool act = g.gl < g.L;
-    unsigned X = group_words<GD>(c, g, xi);
-    unsigned D = group_words<GD>(c, g, di);
+    unsigned X, D;
+    group_words_pair<GD>(c, g, xi, di, X, D);
     const int wx = wlen<GD>(g, X), wd = wlen<GD>(g, D);
     q = 0;
     r = xi;
