@@ -290,9 +290,8 @@ constexpr unsigned RARE_CAP = 1u << 20;  // queued rare items per stream (8 MiB 
 constexpr unsigned RARE_RES_CAP = 1u << 16;  // of which staged with their residues (32 MiB)
 
 // The rare-item queue of `stream` (allocated and cleared on the first call on that stream).  The
-// counters are first cleared by a kernel on `stream` itself (a hipMemset is ordered on the null
-// stream only, and the kernels of a non-blocking stream could otherwise read the counters before
-// it); after that, each rare pass clears them when its last block finishes.
+// counters are cleared by a kernel on `stream` itself: a hipMemset is ordered on the null stream
+// only, and the kernels of a non-blocking stream could otherwise read the counters before it.
 grns_status rare_queue(grns_ctx *c, void *stream, RareQ &q) {
     std::lock_guard<std::mutex> lock(c->qmu);
     for (auto &e : c->queues)
@@ -581,6 +580,7 @@ grns_status grns_eval(grns_ctx *c, const int *x, long long smod, long long snum,
         else hipLaunchKernelGGL((k_eval_rare<NN, false>), rg, blk, 0, S(stream), c->d, x, smod, snum, cnt, o,      \
                                 c->d_status, q);                                                                 \
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                                \
+        hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), q.ctr);                                \
         return launch_check();                                                                                   \
     }
     switch (c->n) {
@@ -633,6 +633,7 @@ grns_status grns_cmp(grns_ctx *c, const int *x, long long xsm, long long xsn, co
         hipLaunchKernelGGL((k_cmp_rare<NN>), rg, blk, 0, S(stream), c->d, x, xsm, xsn, y, ysm, ysn, cnt, sign,       \
                            c->d_status, q);                                                                        \
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;                                                  \
+        hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), q.ctr);                                  \
         return launch_check();                                                                                     \
     }
     switch (c->n) {
@@ -695,6 +696,7 @@ grns_status grns_div(grns_ctx *c, const int *x, const int *d, long long smod, lo
         if (fast) hipLaunchKernelGGL((k_div_rare<true>), rg, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, c->d_status, rq);
         else hipLaunchKernelGGL((k_div_rare<false>), rg, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, c->d_status, rq);
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;
+        hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), rq.ctr);
         return launch_check();
     }
     if (fast) hipLaunchKernelGGL((k_div<true>), grid, blk, 0, S(stream), c->d, G, x, d, smod, snum, cnt, q, r, c->d_status);
@@ -810,6 +812,7 @@ grns_status grns_scale2pow(grns_ctx *c, const int *x, long long smod, long long 
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;
         hipLaunchKernelGGL(k_scale2pow_rare, rg, blk, 0, S(stream), c->d, G, x, smod, snum, D, cnt, y, rq);
         if (hipGetLastError() != hipSuccess) return GRNS_ERR_HIP;
+        hipLaunchKernelGGL(k_rare_reset, dim3(1), dim3(64), 0, S(stream), rq.ctr);
         return launch_check();
     }
     hipLaunchKernelGGL(k_scale2pow, grid, blk, 0, S(stream), c->d, G, x, smod, snum, D, cnt, y);
