@@ -217,6 +217,29 @@ GRNS_HD double fast_scalbn(double x, int n) {
     return bitsd(u & (1ull << 63));  // signed zero
 }
 
+// Division of a two-word number <u1, u0> = u1 2^32 + u0 by a normalised one-word d (top bit set,
+// u1 < d) with a precomputed reciprocal v = floor((2^64 - 1) / d) - 2^32: Moller and Granlund,
+// "Improved division by invariant integers", IEEE Trans. Computers 60(2), 2011, Algorithm 4.  One
+// product, a two-word addition and two conditional corrections, all integer: with wave-uniform
+// operands it runs on the scalar unit.  Returns the quotient (< 2^32); r gets the remainder.
+GRNS_HD unsigned recip_preinv(unsigned d) { return (unsigned)(0xFFFFFFFFFFFFFFFFull / d - (1ull << 32)); }
+GRNS_HD unsigned div21_preinv(unsigned u1, unsigned u0, unsigned d, unsigned v, unsigned &r) {
+    const unsigned long long p = (unsigned long long)v * u1 + (((unsigned long long)u1 << 32) | u0);
+    unsigned q1 = (unsigned)(p >> 32) + 1u;
+    const unsigned q0 = (unsigned)p;
+    unsigned rr = u0 - q1 * d;  // mod 2^32
+    if (rr > q0) {
+        q1 -= 1u;
+        rr += d;
+    }
+    if (rr >= d) {  // rare
+        q1 += 1u;
+        rr -= d;
+    }
+    r = rr;
+    return q1;
+}
+
 // x86-64 (unsigned int)double (cvttsd2si to 64 bits, low 32 kept), for |x| < 2^31.
 GRNS_HD unsigned to_uint(double x) { return (unsigned)(int)x; }
 
@@ -907,7 +930,7 @@ GRNS_D void group_div_words(const DevCtx *c, const Group &g, int xi, int di, int
     }
     const unsigned long long v1 = gword<GD>(g, D, wd - 1);
     const unsigned long long v2 = wd >= 2 ? gword<GD>(g, D, wd - 2) : 0u;
-    const double rv1 = 1.0 / (double)v1;  // the quotient estimate is corrected to the exact floor below
+    const unsigned vinv = recip_preinv((unsigned)v1);  // v1 is normalised: its top bit is set
     const int top = wx - wd;
     unsigned Ds = wshift_up<GD>(g, D, top);
     unsigned Q = 0;
@@ -919,13 +942,10 @@ GRNS_D void group_div_words(const DevCtx *c, const Group &g, int xi, int di, int
         if (u2 >= v1) {
             qh = 0xffffffffull;
             rh = num - qh * v1;
-        } else {
-            qh = (unsigned long long)(((double)u2 * 0x1p32 + (double)u1) * rv1);
-            if (qh > 0xffffffffull) qh = 0xffffffffull;
-            long long rr = (long long)(num - qh * v1);
-            while (rr < 0) { qh--; rr += (long long)v1; }
-            while (rr >= (long long)v1) { qh++; rr -= (long long)v1; }
-            rh = (unsigned long long)rr;
+        } else {  // the exact floor of <u2, u1> / v1 and its remainder
+            unsigned r32;
+            qh = div21_preinv((unsigned)u2, (unsigned)u1, (unsigned)v1, vinv, r32);
+            rh = r32;
         }
         while (rh < (1ull << 32) && qh * v2 > ((rh << 32) | u0)) {
             qh--;

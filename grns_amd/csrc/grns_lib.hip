@@ -1329,6 +1329,12 @@ int grns_debug_mulmod_recip(int x, int y, int m, double d) { return mulmod_recip
 int grns_debug_addmod_recip(int x, int y, int m, double d) { return addmod_recip(x, y, m, d); }
 double grns_debug_bn_err(double c) { return bn_err(c); }
 double grns_debug_fast_scalbn(double x, int n) { return fast_scalbn(x, n); }
+unsigned grns_debug_div21(unsigned u1, unsigned u0, unsigned d, unsigned *r) {
+    unsigned rr = 0;
+    const unsigned q = div21_preinv(u1, u0, d, recip_preinv(d), rr);
+    if (r) *r = rr;
+    return q;
+}
 double grns_debug_filter_tau(const grns_ctx *c) { return c->h.tau; }
 double grns_debug_filter_err(const grns_ctx *c) { return c->h.err; }
 }

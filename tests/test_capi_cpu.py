@@ -299,3 +299,25 @@ def test_rare_lane_routines_match_oracle(n):
             dc += 1
             assert s == o.cmp1(xr, yr), (a, b)
     assert dc > 250
+
+
+def test_word_division_step_is_exact():
+    """The word division's 2/1 step (a reciprocal multiply and two corrections) returns
+    divmod(u1 2^32 + u0, d) for every normalised d and u1 < d: the extremes of d, u1 and u0 and
+    random values, checked against Python's integer division."""
+    import ctypes as C
+    L = G.lib()
+    rnd = random.Random(21)
+    B = 1 << 32
+    ds = [1 << 31, (1 << 31) + 1, B - 1, B - 2, 3 << 30, 0xAAAAAAAB] + [rnd.randrange(1 << 31, B) for _ in range(300)]
+    r = C.c_uint()
+    checked = 0
+    for d in ds:
+        u1s = [0, 1, d - 1, d - 2, d // 2] + [rnd.randrange(d) for _ in range(20)]
+        u0s = [0, 1, B - 1, B - 2, d - 1, d, (d + 1) % B] + [rnd.randrange(B) for _ in range(8)]
+        for u1 in u1s:
+            for u0 in u0s:
+                q = L.grns_debug_div21(u1, u0, d, C.byref(r))
+                assert (q, r.value) == divmod(u1 * B + u0, d), (u1, u0, d)
+                checked += 1
+    assert checked > 100000
