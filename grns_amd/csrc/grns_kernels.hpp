@@ -222,12 +222,13 @@ GRNS_D bool rare_push_staged(const RareQ &q, bool rare, long long j, const int (
     const unsigned slot = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
     if (slot < q.cap) {
         if (slot < q.res_cap) {
-            int *r = q.res + (size_t)slot * RARE_RES_WIDTH;
+            // 16-byte stores: a lane's copy is N/4 (2N/4) write transactions, not N (2N)
+            int4 *r = reinterpret_cast<int4 *>(q.res + (size_t)slot * RARE_RES_WIDTH);
 #pragma unroll
-            for (int i = 0; i < N; i++) r[i] = xr[i];
+            for (int i = 0; i < N; i += 4) r[i / 4] = make_int4(xr[i], xr[i + 1], xr[i + 2], xr[i + 3]);
             if constexpr (PAIR) {
 #pragma unroll
-                for (int i = 0; i < N; i++) r[N + i] = yr[i];
+                for (int i = 0; i < N; i += 4) r[(N + i) / 4] = make_int4(yr[i], yr[i + 1], yr[i + 2], yr[i + 3]);
             }
         }
         q.items[slot] = j;

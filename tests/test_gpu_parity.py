@@ -1043,3 +1043,24 @@ def test_mpint_and_max_other_sizes(dev, n):
         assert (Z[0].cpu().numpy() == rd).all() and (Z[1].cpu().numpy() == rs).all(), op
         assert Z[2].cpu().numpy().tobytes() == re.tobytes(), op
     assert c.status() == 0
+
+
+@pytest.mark.parametrize("n", [8, 64])
+def test_streaming_kernels_persistent_grid(ctxs, dev, n):
+    """Grid-stride loops that take several passes per block: eval (a persistent grid of one block
+    per CU by default at n = 64) and cmp under a cap of one block per CU, on 2^17 + 77 numbers,
+    against the oracle."""
+    c, o, mod = ctxs[n]
+    cnt = (1 << 17) + 77
+    vals = eval_inputs(mod, cnt, seed=90 + n)[:cnt]
+    xs, ys = (v[:cnt] for v in cmp_pairs(mod, cnt, seed=91 + n))
+    x, xa, ya = to_res(vals, mod), to_res(xs, mod), to_res(ys, mod)
+    try:
+        c.grid_cap(1)
+        assert _intervals_bytes(c.eval(_dev(x, dev))) == o.eval_array(x).tobytes()
+        assert (c.cmp(_dev(xa, dev), _dev(ya, dev)).cpu().numpy() == o.cmp_array(xa, ya)).all()
+        c.grid_cap(0)
+        assert _intervals_bytes(c.eval(_dev(x, dev))) == o.eval_array(x).tobytes()
+        assert c.status() == 0
+    finally:
+        c.grid_cap(0)
