@@ -86,7 +86,7 @@ DEBUG_EXPORTS = [
     "grns_debug_div21",
     "grns_debug_filter_tau", "grns_debug_filter_err", "grns_debug_stage_timing", "grns_debug_stage_read",
     "grns_debug_reference_loops", "grns_debug_stream_tiles", "grns_debug_grid_cap", "grns_debug_batch_per_call",
-    "grns_debug_rare_staging",
+    "grns_debug_rare_staging", "grns_debug_rare_spread",
 ]
 
 
@@ -162,6 +162,7 @@ def lib():
     L.grns_debug_stream_tiles.argtypes = [P, I]
     L.grns_debug_grid_cap.argtypes = [P, I]
     L.grns_debug_rare_staging.argtypes = [P, I]
+    L.grns_debug_rare_spread.argtypes = [P, I]
     L.grns_debug_batch_per_call.argtypes = [P, P]
     L.grns_debug_stage_read.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     _lib = L
@@ -390,6 +391,10 @@ class Context:
     def rare_staging(self, on=True):
         """Stage queued numbers' residues for the rare pass (on, the default) or not (test hook)."""
         _check(lib().grns_debug_rare_staging(self._h, int(on)), "grns_debug_rare_staging")
+
+    def rare_spread(self, on=True):
+        """Spread rare items over the sub-queues (on, the default) or use one queue (test hook)."""
+        _check(lib().grns_debug_rare_spread(self._h, int(on)), "grns_debug_rare_spread")
 
     def reference_loops(self, on=True):
         """Division and scaling through the reference's loops for every number (test hook)."""

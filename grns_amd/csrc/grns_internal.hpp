@@ -42,6 +42,7 @@ struct grns_ctx {
     int stream_mode = 0;            // eval: 0 direct loads, 1 LDS-staged tiles, 2 next number prefetched (test hook)
     bool reference_loops = false;  // division and scaling: the reference loops for every number (test hook)
     bool rare_staging = true;      // rare queues stage the residues of queued numbers (test hook)
+    bool rare_spread = true;       // rare items spread over the sub-queues (test hook)
     int cus = 0;                   // compute units of the device
     int grid_cap = 0;              // streaming kernels: at most grid_cap x cus blocks (0: one per BLOCK numbers)
     StageTimer timers[3];

@@ -169,73 +169,118 @@ GRNS_D void load_number(const int *__restrict__ x, long long smod, long long snu
 // algorithm to a second kernel: the streaming pass appends their indices here, and the rare pass
 // spreads them over every wave group of the device, so that a few long evaluations (refinement
 // loops, mixed-radix conversions) neither stall the streaming waves nor run one after another in
-// the wave that met them.  ctr[0] counts pushes and ctr[2] is set when the queue was full; then the
-// overflowing items carry a marker in their output and the rare pass scans for it.  The
-// thread-per-number kernels also copy a queued number's residues (a pair's, for the comparison)
-// from registers into `res`, RARE_RES_WIDTH ints per slot for the first res_cap slots, so that the
+// the wave that met them.  The queue is split into RARE_SUBQ sub-queues, each with its own counter
+// on its own 256-byte line (block b pushes to sub-queue b mod RARE_SUBQ): device-scope atomics on
+// one address are served one after another, and with tens of thousands of rare lanes that alone
+// outlasts a fast streaming pass.  ctr[2] is set when a sub-queue was full; then the overflowing
+// items carry a marker in their output and the rare pass scans for it.  The thread-per-number
+// kernels also copy a queued number's residues (a pair's, for the comparison) from registers into
+// `res`, RARE_RES_WIDTH ints per slot for the first res_cap slots of each sub-queue, so that the
 // rare pass reads 4n contiguous bytes per item instead of one cache line per residue row.  A
 // one-wave kernel after the rare pass clears the counters for the next call on the stream.
 // Evaluation, comparison, division and scaling share the queue of a (context, stream) pair, one
 // call at a time.
 constexpr int RARE_RES_WIDTH = 128;  // ints per staged slot: two numbers of up to 64 residues
+constexpr int RARE_SUBQ = 64;        // sub-queues
+constexpr int RARE_SUB_BASE = 64;    // ctr[RARE_SUB_BASE + s * RARE_SUB_STRIDE]: count of sub-queue s
+constexpr int RARE_SUB_STRIDE = 64;  // 256 bytes between counters
+constexpr int RARE_CTR_WORDS = RARE_SUB_BASE + RARE_SUBQ * RARE_SUB_STRIDE;
 struct RareQ {
-    long long *items;
+    long long *items;  // sub-queue s holds items[s * cap .. s * cap + cap)
     unsigned *ctr;
-    unsigned cap;
-    int *res;
+    unsigned cap;      // items per sub-queue
+    int *res;          // staged residues of the first res_cap items of each sub-queue
     unsigned res_cap;
+    unsigned submask;  // block b pushes to sub-queue b & submask
 };
 
-// Appends j for the lanes with `rare` (all 64 lanes of the wave call it, one atomic per wave).
-// Returns false for a lane whose item did not fit (the caller then writes the marker).
-GRNS_D bool rare_push(const RareQ &q, bool rare, long long j) {
+GRNS_D unsigned *rare_counter(const RareQ &q, int s) { return q.ctr + RARE_SUB_BASE + s * RARE_SUB_STRIDE; }
+
+// Reserves slots in the block's sub-queue for the lanes with `rare` (all 64 lanes of the wave call
+// it, one atomic per wave).  Returns the lane's slot within the sub-queue, or ~0u when it did not
+// fit (the overflow flag is then set and the caller writes the marker); 0 for the other lanes.
+GRNS_D unsigned rare_reserve(const RareQ &q, bool rare, int &sub) {
     const unsigned long long m = __ballot(rare);
-    if (!m) return true;
+    sub = (int)(blockIdx.x & q.submask);
+    if (!m) return 0u;
     const int lane = threadIdx.x & 63;
     const int leader = __ffsll((long long)m) - 1;
     unsigned base = 0;
-    if (lane == leader) base = atomicAdd(&q.ctr[0], (unsigned)__popcll(m));
+    if (lane == leader) base = atomicAdd(rare_counter(q, sub), (unsigned)__popcll(m));
     base = __shfl(base, leader, 64);
-    if (!rare) return true;
+    if (!rare) return 0u;
     const unsigned slot = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-    if (slot < q.cap) {
-        q.items[slot] = j;
-        return true;
-    }
+    if (slot < q.cap) return slot;
     q.ctr[2] = 1u;
-    return false;
+    return ~0u;
+}
+
+// Appends j for the lanes with `rare`.  Returns false for a lane whose item did not fit.
+GRNS_D bool rare_push(const RareQ &q, bool rare, long long j) {
+    int sub;
+    const unsigned slot = rare_reserve(q, rare, sub);
+    if (!rare) return true;
+    if (slot == ~0u) return false;
+    q.items[(size_t)sub * q.cap + slot] = j;
+    return true;
 }
 
 // rare_push that also stages the residues of the queued numbers: xr (and yr, for a pair: PAIR) of
-// the lanes whose slot is below res_cap, at res[slot * RARE_RES_WIDTH] (x) and + N (y).  The
-// arrays stay register arrays (references, indices known at compile time).
+// the lanes whose slot is below res_cap, at res[(sub * res_cap + slot) * RARE_RES_WIDTH] (x) and
+// + N (y), as 16-byte stores.  The arrays stay register arrays (references, indices known at
+// compile time).
 template <int N, bool PAIR>
 GRNS_D bool rare_push_staged(const RareQ &q, bool rare, long long j, const int (&xr)[N], const int (&yr)[N]) {
-    const unsigned long long m = __ballot(rare);
-    if (!m) return true;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)m) - 1;
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(&q.ctr[0], (unsigned)__popcll(m));
-    base = __shfl(base, leader, 64);
+    int sub;
+    const unsigned slot = rare_reserve(q, rare, sub);
     if (!rare) return true;
-    const unsigned slot = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-    if (slot < q.cap) {
-        if (slot < q.res_cap) {
-            // 16-byte stores: a lane's copy is N/4 (2N/4) write transactions, not N (2N)
-            int4 *r = reinterpret_cast<int4 *>(q.res + (size_t)slot * RARE_RES_WIDTH);
+    if (slot == ~0u) return false;
+    if (slot < q.res_cap) {
+        int4 *r = reinterpret_cast<int4 *>(q.res + ((size_t)sub * q.res_cap + slot) * RARE_RES_WIDTH);
 #pragma unroll
-            for (int i = 0; i < N; i += 4) r[i / 4] = make_int4(xr[i], xr[i + 1], xr[i + 2], xr[i + 3]);
-            if constexpr (PAIR) {
+        for (int i = 0; i < N; i += 4) r[i / 4] = make_int4(xr[i], xr[i + 1], xr[i + 2], xr[i + 3]);
+        if constexpr (PAIR) {
 #pragma unroll
-                for (int i = 0; i < N; i += 4) r[(N + i) / 4] = make_int4(yr[i], yr[i + 1], yr[i + 2], yr[i + 3]);
-            }
+            for (int i = 0; i < N; i += 4) r[(N + i) / 4] = make_int4(yr[i], yr[i + 1], yr[i + 2], yr[i + 3]);
         }
-        q.items[slot] = j;
-        return true;
     }
-    q.ctr[2] = 1u;
-    return false;
+    q.items[(size_t)sub * q.cap + slot] = j;
+    return true;
+}
+
+// The rare pass's view of the queue: the items of the sub-queues laid end to end (item k of the
+// pass is slot k - pre[s] of the sub-queue s with pre[s] <= k < pre[s + 1]), in LDS for the block.
+struct RareView {
+    unsigned pre[RARE_SUBQ + 1];
+};
+
+// Every thread of the block calls it (a barrier inside): the prefix of the sub-queue counts.
+GRNS_D void rare_view(const RareQ &q, RareView &v) {
+    if (threadIdx.x < 64) {
+        const int s = threadIdx.x;
+        const unsigned cs = *rare_counter(q, s);
+        unsigned inc = cs < q.cap ? cs : q.cap;
+        for (int h = 1; h < 64; h <<= 1) {
+            const unsigned t = (unsigned)__shfl_up((int)inc, h, 64);
+            if (s >= h) inc += t;
+        }
+        v.pre[s + 1] = inc;
+        if (s == 0) v.pre[0] = 0u;
+    }
+    __syncthreads();
+}
+
+// Item k < pre[RARE_SUBQ] of the pass: its index j, and its staged residues or nullptr.
+GRNS_D long long rare_locate(const RareQ &q, const RareView &v, long long k, const int *&staged) {
+    int lo = 0, hi = RARE_SUBQ;  // pre[lo] <= k < pre[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((long long)v.pre[mid] <= k) lo = mid;
+        else hi = mid;
+    }
+    const unsigned slot = (unsigned)(k - v.pre[lo]);
+    staged = slot < q.res_cap ? q.res + ((size_t)lo * q.res_cap + slot) * RARE_RES_WIDTH : nullptr;
+    return q.items[(size_t)lo * q.cap + slot];
 }
 
 // Clears the counters of a rare-item queue for the next call on its stream (one wave, launched
@@ -244,6 +289,7 @@ GRNS_D bool rare_push_staged(const RareQ &q, bool rare, long long j, const int (
 // over 2^24 numbers, against 4.5 us for this kernel.
 static __global__ void k_rare_reset(unsigned *ctr) {
     if (threadIdx.x < 4) ctr[threadIdx.x] = 0u;
+    if (threadIdx.x < RARE_SUBQ) ctr[RARE_SUB_BASE + threadIdx.x * RARE_SUB_STRIDE] = 0u;
 }
 
 // Marker of an evaluation left to the rare pass after a queue overflow (no result has it).
@@ -930,11 +976,13 @@ GRNS_HD int cmp_full(const DevCtx *c, const int (&xr)[N], const int (&yr)[N], co
     return (Tx > Ty) - (Tx < Ty);
 }
 
-// Item k of a rare pass: the k-th queued index, then (after an overflow) output positions, of
-// which only the marked ones are taken.  -1 for none.
-GRNS_D long long rare_item(const RareQ &q, long long nq, long long k, long long total) {
-    if (k < nq) return q.items[k];
-    return k < total ? k - nq : -1;
+// Item k of a rare pass: the queued items (k < nq = pre[RARE_SUBQ]), then (after an overflow)
+// every output position k - nq, of which only the marked ones are taken.  staged: the item's
+// staged residues, or nullptr.
+GRNS_D long long rare_item(const RareQ &q, const RareView &v, long long k, const int *&staged) {
+    staged = nullptr;
+    if (k < (long long)v.pre[RARE_SUBQ]) return rare_locate(q, v, k, staged);
+    return k - (long long)v.pre[RARE_SUBQ];
 }
 
 // Rare pass of the evaluation: one wave group per item (the group forms settle the ambiguous
@@ -948,14 +996,15 @@ __global__ __launch_bounds__(BLOCK) void k_eval_rare(const DevCtx *__restrict__ 
     const int P = 64 / G;
     const long long gid = (((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6) * P + (threadIdx.x & 63) / G;
     const long long ngroups = ((long long)gridDim.x * BLOCK >> 6) * P;
-    const unsigned pushed = q.ctr[0];
-    const long long nq = pushed < q.cap ? pushed : q.cap;
+    __shared__ RareView v;
+    rare_view(q, v);
+    const long long nq = v.pre[RARE_SUBQ];
     const long long total = nq + (q.ctr[2] ? count : 0);
     int st = 0;
     for (long long k = gid; k < total; k += ngroups) {
-        const long long j = rare_item(q, nq, k, total);
+        const int *src;
+        const long long j = rare_item(q, v, k, src);
         if (k >= nq && reinterpret_cast<const long long *>(out + j)[1] != EVAL_MARK) continue;
-        const int *src = k < (long long)q.res_cap && k < nq ? q.res + k * RARE_RES_WIDTH : nullptr;
         const int xi = g.gl < N ? (src ? src[g.gl] : x[j * snum + (long long)g.gl * smod]) : 0;
         interval_t r = group_eval<FAST>(c, g, xi, st);
         if (g.gl == 0) store_interval(out + j, r);
@@ -977,14 +1026,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     const int P = 64 / G;
     const long long gid = (((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6) * P + (threadIdx.x & 63) / G;
     const long long ngroups = ((long long)gridDim.x * BLOCK >> 6) * P;
-    const unsigned pushed = q.ctr[0];
-    const long long nq = pushed < q.cap ? pushed : q.cap;
+    __shared__ RareView v;
+    rare_view(q, v);
+    const long long nq = v.pre[RARE_SUBQ];
     const long long total = nq + (q.ctr[2] ? count : 0);
     int st = 0;
     for (long long k = gid; k < total; k += ngroups) {
-        const long long j = rare_item(q, nq, k, total);
+        const int *src;
+        const long long j = rare_item(q, v, k, src);
         if (k >= nq && sign[j] != 2) continue;
-        const int *src = k < (long long)q.res_cap && k < nq ? q.res + k * RARE_RES_WIDTH : nullptr;
         const int xi = g.gl < N ? (src ? src[g.gl] : x[j * xsn + (long long)g.gl * xsm]) : 0;
         const int yi = g.gl < N ? (src ? src[N + g.gl] : y[j * ysn + (long long)g.gl * ysm]) : 0;
         const int r = group_cmp(c, g, xi, yi, st);
@@ -1689,12 +1739,14 @@ __global__ __launch_bounds__(BLOCK) void k_div_rare(const DevCtx *__restrict__ c
     const int P = 64 / G;
     const long long gid = (((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6) * P + (threadIdx.x & 63) / G;
     const long long ngroups = ((long long)gridDim.x * BLOCK >> 6) * P;
-    const unsigned pushed = rq.ctr[0];
-    const long long nq = pushed < rq.cap ? pushed : rq.cap;
+    __shared__ RareView v;
+    rare_view(rq, v);
+    const long long nq = v.pre[RARE_SUBQ];
     const long long total = nq + (rq.ctr[2] ? count : 0);
     int st = 0;
     for (long long k = gid; k < total; k += ngroups) {
-        const long long j = rare_item(rq, nq, k, total);
+        const int *unused;
+        const long long j = rare_item(rq, v, k, unused);
         if (k >= nq && q[j * snum] != DIV_MARK) continue;
         const long long off = j * snum + (long long)g.gl * smod;
         const int xi = g.gl < g.L ? x[off] : 0;
@@ -1993,11 +2045,13 @@ static __global__ __launch_bounds__(BLOCK) void k_scale2pow_rare(const DevCtx *_
     const int P = 64 / G;
     const long long gid = (((long long)blockIdx.x * BLOCK + threadIdx.x) >> 6) * P + (threadIdx.x & 63) / G;
     const long long ngroups = ((long long)gridDim.x * BLOCK >> 6) * P;
-    const unsigned pushed = rq.ctr[0];
-    const long long nq = pushed < rq.cap ? pushed : rq.cap;
+    __shared__ RareView v;
+    rare_view(rq, v);
+    const long long nq = v.pre[RARE_SUBQ];
     const long long total = nq + (rq.ctr[2] ? count : 0);
     for (long long k = gid; k < total; k += ngroups) {
-        const long long j = rare_item(rq, nq, k, total);
+        const int *unused;
+        const long long j = rare_item(rq, v, k, unused);
         if (k >= nq && y[j * snum] != DIV_MARK) continue;
         const long long off = j * snum + (long long)g.gl * smod;
         const int xi = g.gl < g.L ? x[off] : 0;

@@ -286,8 +286,17 @@ struct DeviceGuard {
     }
 };
 
-constexpr unsigned RARE_CAP = 1u << 20;  // queued rare items per stream (8 MiB of indices)
+constexpr unsigned RARE_CAP = 1u << 20;      // queued rare items per stream (8 MiB of indices)
 constexpr unsigned RARE_RES_CAP = 1u << 16;  // of which staged with their residues (32 MiB)
+
+// The queue's split: RARE_SUBQ sub-queues of RARE_CAP / RARE_SUBQ items (or, with the test hook
+// off, all of it in sub-queue 0).
+void rare_split(const grns_ctx *c, RareQ &q) {
+    const unsigned parts = c->rare_spread ? (unsigned)RARE_SUBQ : 1u;
+    q.cap = RARE_CAP / parts;
+    q.res_cap = c->rare_staging ? RARE_RES_CAP / parts : 0u;
+    q.submask = parts - 1u;
+}
 
 // The rare-item queue of `stream` (allocated and cleared on the first call on that stream).  The
 // counters are cleared by a kernel on `stream` itself: a hipMemset is ordered on the null stream
@@ -298,14 +307,13 @@ grns_status rare_queue(grns_ctx *c, void *stream, RareQ &q) {
         if (e.stream == stream) {
             q.items = e.items;
             q.ctr = e.ctr;
-            q.cap = RARE_CAP;
             q.res = e.res;
-            q.res_cap = c->rare_staging ? RARE_RES_CAP : 0u;
+            rare_split(c, q);
             return GRNS_OK;
         }
     grns_ctx::RareQueue e{stream, nullptr, nullptr, nullptr};
     if (hipMalloc(&e.items, sizeof(long long) * RARE_CAP) != hipSuccess) return GRNS_ERR_OOM;
-    if (hipMalloc(&e.ctr, 4 * sizeof(unsigned)) != hipSuccess ||
+    if (hipMalloc(&e.ctr, RARE_CTR_WORDS * sizeof(unsigned)) != hipSuccess ||
         hipMalloc(&e.res, sizeof(int) * RARE_RES_WIDTH * (size_t)RARE_RES_CAP) != hipSuccess) {
         (void)hipFree(e.items);
         if (e.ctr) (void)hipFree(e.ctr);
@@ -321,9 +329,8 @@ grns_status rare_queue(grns_ctx *c, void *stream, RareQ &q) {
     c->queues.push_back(e);
     q.items = e.items;
     q.ctr = e.ctr;
-    q.cap = RARE_CAP;
     q.res = e.res;
-    q.res_cap = c->rare_staging ? RARE_RES_CAP : 0u;
+    rare_split(c, q);
     return GRNS_OK;
 }
 
@@ -1296,6 +1303,12 @@ grns_status grns_debug_stage_read(grns_ctx *c, int op, double *ms_total, long lo
     *launches = t.launches;
     t.ms = 0.0;
     t.launches = 0;
+    return GRNS_OK;
+}
+
+grns_status grns_debug_rare_spread(grns_ctx *c, int on) {
+    if (!c) return GRNS_ERR_ARG;
+    c->rare_spread = on != 0;
     return GRNS_OK;
 }
 

@@ -51,6 +51,10 @@ grns_status grns_debug_grid_cap(grns_ctx *ctx, int blocks_per_cu);
  * of the numbers they queue for the rare pass (which then reads them from the inputs); 1, the
  * default, restores it.  Results do not depend on it. */
 grns_status grns_debug_rare_staging(grns_ctx *ctx, int on);
+/* Test hook: on = 0 puts every rare item of a call into one queue with one atomic counter for the
+ * device; 1, the default, spreads them over sub-queues with a counter each, by block.  Results do
+ * not depend on it. */
+grns_status grns_debug_rare_spread(grns_ctx *ctx, int on);
 /* Baseline of the batched form: runs every task of the batch through its per-call entry point
  * (grns_eval, grns_cmp, grns_mpint_*, grns_div, grns_scale2pow), one call per task in table order
  * on `stream`, from C++ -- what a caller without the batched form does.  Results equal those of

@@ -543,14 +543,16 @@ def test_rare_queue_overflow(ctxs, dev, n):
 
 
 @pytest.mark.parametrize("n", [8, 32])
-@pytest.mark.parametrize("staging", [True, False])
-def test_rare_items_distinct(ctxs, dev, n, staging):
+@pytest.mark.parametrize("staging,spread", [(True, True), (False, True), (True, False)])
+def test_rare_items_distinct(ctxs, dev, n, staging, spread):
     """2^17 + 999 distinct numbers just below M in shuffled order (each needs the rare pass of
     evaluation and of comparison): every queued item is resolved for its own number (the first
-    2^16 from residues staged by the streaming pass, the rest, or all of them with staging off,
-    from the inputs); every interval (as bytes) and sign equals the CPU path's."""
+    2^16 of the queue from residues staged by the streaming pass, the rest, or all of them with
+    staging off, from the inputs), with the items spread over the sub-queues or all in one; every
+    interval (as bytes) and sign equals the CPU path's."""
     c, o, mod = ctxs[n]
     c.rare_staging(staging)
+    c.rare_spread(spread)
     from grns_amd.moduli import product
     M = product(mod)
     count = (1 << 17) + 999
@@ -564,6 +566,7 @@ def test_rare_items_distinct(ctxs, dev, n, staging):
     assert _intervals_bytes(c.eval(xd)) == o.eval_array(xa).tobytes()
     assert (c.cmp(xd, yd).cpu().numpy() == o.cmp_array(xa, ya)).all()
     c.rare_staging(True)
+    c.rare_spread(True)
     assert c.status() == 0
 
 
