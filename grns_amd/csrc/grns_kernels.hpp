@@ -1012,12 +1012,9 @@ __global__ __launch_bounds__(BLOCK) void k_eval_rare(const DevCtx *__restrict__ 
     if (g.gl == 0 && st) atomicOr(status, st);
 }
 
-// Rare pass of the comparison: one wave group per item.  Six waves per SIMD: the items are
-// latency-bound group computations, and more of them in flight beat the few registers spilled
-// (75.6 against 81.7 us for the comparison workload's 35k pairs at n = 32; the evaluation's pass
-// was slower that way, 29.0 against 23.2 us, and keeps its registers).
+// Rare pass of the comparison: one wave group per item.
 template <int N>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_cmp_rare(const DevCtx *__restrict__ c, const int *__restrict__ x,
+__global__ __launch_bounds__(BLOCK) void k_cmp_rare(const DevCtx *__restrict__ c, const int *__restrict__ x,
                                                     long long xsm, long long xsn, const int *__restrict__ y,
                                                     long long ysm, long long ysn, long long count,
                                                     int *__restrict__ sign, int *__restrict__ status, RareQ q) {
