@@ -529,8 +529,8 @@ def main():
         traffic_from_counters(*args.traffic)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the RNS path shards "
-              "element ranges across GPUs with no exchange except the arg-max winners; see DESIGN.md).",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the RNS path would shard "
+              "element ranges across GPUs, one process per GPU, exchanging only the arg-max winners).",
               file=sys.stderr)
         sys.exit(2)
     if args.pmc_workload:
