@@ -85,7 +85,7 @@ DEBUG_EXPORTS = [
     "grns_debug_mulmod_recip", "grns_debug_addmod_recip", "grns_debug_bn_err", "grns_debug_fast_scalbn",
     "grns_debug_div21",
     "grns_debug_filter_tau", "grns_debug_filter_err", "grns_debug_stage_timing", "grns_debug_stage_read",
-    "grns_debug_reference_loops", "grns_debug_stream_tiles", "grns_debug_grid_cap", "grns_debug_batch_per_call",
+    "grns_debug_reference_loops", "grns_debug_grid_cap", "grns_debug_batch_per_call",
     "grns_debug_rare_staging", "grns_debug_rare_spread",
 ]
 
@@ -159,7 +159,6 @@ def lib():
     L.grns_debug_filter_err.argtypes = [P]
     L.grns_debug_stage_timing.argtypes = [P, I]
     L.grns_debug_reference_loops.argtypes = [P, I]
-    L.grns_debug_stream_tiles.argtypes = [P, I]
     L.grns_debug_grid_cap.argtypes = [P, I]
     L.grns_debug_rare_staging.argtypes = [P, I]
     L.grns_debug_rare_spread.argtypes = [P, I]
@@ -378,11 +377,6 @@ class Context:
         xa, ya, za = mpint_array(*x), mpint_array(*y), mpint_array(*z)
         _check(fn(self._h, C.byref(xa), C.byref(ya), C.byref(za), count,
                   _tptr(overflow) if overflow is not None else None, _stream_ptr(stream)), f"grns_mpint_{op}")
-
-    def stream_tiles(self, mode=1):
-        """How eval's streaming kernel loads residues (test hook): 0 direct (the default), 1 LDS
-        tiles, 2 next number prefetched into registers."""
-        _check(lib().grns_debug_stream_tiles(self._h, int(mode)), "grns_debug_stream_tiles")
 
     def grid_cap(self, blocks_per_cu=0):
         """Cap the streaming kernels' grids at blocks_per_cu blocks per CU, 0 for none (test hook)."""

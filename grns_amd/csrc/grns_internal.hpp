@@ -39,7 +39,6 @@ struct grns_ctx {
         long long launches = 0;
     };
     bool timing = false;
-    int stream_mode = 0;            // eval: 0 direct loads, 1 LDS-staged tiles, 2 next number prefetched (test hook)
     bool reference_loops = false;  // division and scaling: the reference loops for every number (test hook)
     bool rare_staging = true;      // rare queues stage the residues of queued numbers (test hook)
     bool rare_spread = true;       // rare items spread over the sub-queues (test hook)

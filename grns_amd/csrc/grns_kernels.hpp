@@ -362,118 +362,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(N <= 32 ?
     }
 }
 
-// ---------------------------------------------------------------- LDS-staged streaming
-// The streaming kernels above load a number's residues into registers and then spend most of
-// their time on its arithmetic; while a wave computes it has nothing in flight, and the loads of the
-// few other waves a CU holds do not keep HBM busy.  The tiled forms stage tiles of BLOCK numbers
-// (N rows of BLOCK residues, row-major in LDS) with asynchronous global->LDS loads
-// (global_load_lds_dwordx4: each lane moves 16 bytes, the wave's 64 lanes one contiguous 1 KiB row
-// segment into consecutive LDS bytes) and load tile k + 1 while tile k is computed from LDS.
-// Two buffers are named separately and the loop is unrolled by two, so that no LDS read can alias
-// the buffer being filled.  Conditions (checked by the launcher): modulus-major rows that are
-// 16-byte aligned (x and mod_stride * 4 multiples of 16).
-typedef __attribute__((address_space(3))) void lds_void;
-typedef const __attribute__((address_space(1))) void glob_void;
-
-template <int N>
-GRNS_D void tile_issue(const int *__restrict__ x, long long smod, long long t, int *buf) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int *base = x + t * BLOCK + lane * 4;
-#pragma unroll
-    for (int r = wave; r < N; r += BLOCK / 64)
-        __builtin_amdgcn_global_load_lds((glob_void *)(base + (long long)r * smod), (lds_void *)(buf + r * BLOCK), 16,
-                                         0, 0);
-}
-
-// Every staged load of this wave done (and its earlier stores), then every wave of the block.
-GRNS_D void tile_wait() {
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-}
-
-template <int N>
-GRNS_D void tile_read(const int *buf, int (&xr)[N]) {
-#pragma unroll
-    for (int i = 0; i < N; i++) xr[i] = buf[i * BLOCK + threadIdx.x];
-}
-
-template <int N, bool FAST>
-GRNS_D void eval_one(const DevCtx *__restrict__ c, const int (&xr)[N], long long j, interval_t *__restrict__ out,
-                     const RareQ &q) {
-    interval_t r;
-    bool rare = false;
-    if (eval_hot<N, FAST>(c, xr, r)) store_interval(out + j, r);
-    else rare = true;
-    if (!rare_push_staged<N, false>(q, rare, j, xr, xr)) {
-        interval_t mk;
-        mk.low.frac = __builtin_nan(""); mk.low.exp = EVAL_MARK;
-        mk.upp.frac = __builtin_nan(""); mk.upp.exp = EVAL_MARK;
-        store_interval(out + j, mk);
-    }
-}
-
-// rns_eval_compute[_fast] of count numbers: `tiles` = count / BLOCK full tiles through LDS, the
-// remainder by the last block with direct loads.  The rare queue and its pass as k_eval.
-template <int N, bool FAST>
-__global__ __launch_bounds__(BLOCK) void k_eval_tiles(const DevCtx *__restrict__ c, const int *__restrict__ x,
-                                                      long long smod, long long count, interval_t *__restrict__ out,
-                                                      RareQ q) {
-    __shared__ int bufA[N * BLOCK];
-    __shared__ int bufB[N * BLOCK];
-    const long long tiles = count / BLOCK;
-    long long t = blockIdx.x;
-    if (t < tiles) tile_issue<N>(x, smod, t, bufA);
-    while (t < tiles) {
-        int xr[N];
-        tile_wait();
-        long long tn = t + gridDim.x;
-        if (tn < tiles) tile_issue<N>(x, smod, tn, bufB);
-        tile_read<N>(bufA, xr);
-        eval_one<N, FAST>(c, xr, t * BLOCK + threadIdx.x, out, q);
-        if (tn >= tiles) break;
-        t = tn;
-        tile_wait();
-        tn = t + gridDim.x;
-        if (tn < tiles) tile_issue<N>(x, smod, tn, bufA);
-        tile_read<N>(bufB, xr);
-        eval_one<N, FAST>(c, xr, t * BLOCK + threadIdx.x, out, q);
-        t = tn;
-    }
-    if (blockIdx.x == gridDim.x - 1) {  // the last count % BLOCK numbers
-        const long long j = tiles * BLOCK + threadIdx.x;
-        if (j < count) {
-            int xr[N];
-            load_number<N, LD_MM>(x, smod, 1, j, xr);
-            eval_one<N, FAST>(c, xr, j, out, q);
-        }
-    }
-}
-
-// rns_eval_compute[_fast] with the residues of the lane's next number loaded while the current one
-// is evaluated (two register sets, named separately and the loop unrolled by two): a persistent
-// grid in which every wave keeps one number's loads in flight at all times.  Modulus-major rows,
-// count <= 2^29, n <= 32 (checked by the launcher; at least 3 waves per SIMD are asked for).
-// The rare queue and its pass as k_eval.
-template <int N, bool FAST>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_eval_pf(const DevCtx *__restrict__ c, const int *__restrict__ x,
-                                                   long long smod, long long count, interval_t *__restrict__ out,
-                                                   RareQ q) {
-    const long long stride = (long long)gridDim.x * BLOCK;
-    long long j = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    const long long j0 = j - threadIdx.x;  // the block's first number: uniform loop bound
-    int xa[N], xb[N];
-    if (j < count) load_number<N, LD_MM>(x, smod, 1, j, xa);
-    for (long long b = j0; b < count; b += 2 * stride) {
-        const long long ja = b + threadIdx.x, jb = ja + stride;
-        if (jb < count) load_number<N, LD_MM>(x, smod, 1, jb, xb);
-        if (ja < count) eval_one<N, FAST>(c, xa, ja, out, q);
-        if (b + stride >= count) break;
-        const long long jc = jb + stride;
-        if (jc < count) load_number<N, LD_MM>(x, smod, 1, jc, xa);
-        if (jb < count) eval_one<N, FAST>(c, xb, jb, out, q);
-    }
-}
-
 // Any n <= 64: one group per number throughout.
 template <bool FAST>
 __global__ __launch_bounds__(BLOCK) void k_eval_group(const DevCtx *__restrict__ c, int G, const int *__restrict__ x,

@@ -207,36 +207,6 @@ grns_status build_constants(grns_ctx *c, const int *moduli, int n) {
 
 hipStream_t S(void *s) { return (hipStream_t)s; }
 
-// The LDS-staged streaming kernels: for n = 16 and 32 (two tiles of n x BLOCK residues fit the
-// LDS of a CU twice over), modulus-major rows aligned to 16 bytes, at least one full tile.
-bool tiles_ok(const grns_ctx *c, int n, const int *x, long long smod, size_t count) {
-    return c->stream_mode == 1 && (n == 16 || n == 32) && ((uintptr_t)x & 15) == 0 && (smod & 3) == 0 &&
-           count >= (size_t)BLOCK;
-}
-
-// Their grid: the blocks resident at once on the device (occupancy x CUs), or fewer when there are
-// fewer tiles.
-long long tiles_grid(const grns_ctx *c, size_t count, const void *kernel) {
-    int cus = 0, per = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)
-        cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, BLOCK, 0) != hipSuccess || per < 1) per = 1;
-    long long tiles = (long long)(count / BLOCK);
-    long long g = (long long)cus * per;
-    return g < tiles ? g : (tiles > 0 ? tiles : 1);
-}
-
-// Grid of the prefetching streaming kernels: the blocks resident at once (occupancy x CUs, or the
-// context's cap per CU when lower), or fewer when there are fewer blocks of numbers.
-long long pf_grid(const grns_ctx *c, size_t count, const void *kernel) {
-    int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, BLOCK, 0) != hipSuccess || per < 1) per = 1;
-    if (c->grid_cap > 0 && c->grid_cap < per) per = c->grid_cap;
-    long long blocks = (long long)((count + BLOCK - 1) / BLOCK);
-    long long g = (long long)c->cus * per;
-    return g < blocks ? g : (blocks > 0 ? blocks : 1);
-}
-
 // Grid of the thread-per-number kernels of the large moduli sets (n > 64): blocks of one wave,
 // at most `cap` of them (their private arrays take up to a few KiB per thread).
 long long grid_serial(size_t count, long long cap = 4096) {
@@ -559,17 +529,7 @@ grns_status grns_eval(grns_ctx *c, const int *x, long long smod, long long snum,
         if (qs != GRNS_OK) return qs;                                                                            \
         dim3 rg((unsigned)rare_grid(count));                                                                     \
         int tk = stage_begin(c, 0, stream);                                                                      \
-        if (mm && tiles_ok(c, NN, x, smod, count)) {                                                             \
-            dim3 tg((unsigned)tiles_grid(c, count, fast ? (const void *)k_eval_tiles<NN, true>                   \
-                                                        : (const void *)k_eval_tiles<NN, false>));                \
-            if (fast) hipLaunchKernelGGL((k_eval_tiles<NN, true>), tg, blk, 0, S(stream), c->d, x, smod, cnt, o, q); \
-            else hipLaunchKernelGGL((k_eval_tiles<NN, false>), tg, blk, 0, S(stream), c->d, x, smod, cnt, o, q);  \
-        } else if (mm && c->stream_mode == 2 && NN <= 32) {                                                      \
-            dim3 pg((unsigned)pf_grid(c, count, fast ? (const void *)k_eval_pf<NN, true>                         \
-                                                     : (const void *)k_eval_pf<NN, false>));                      \
-            if (fast) hipLaunchKernelGGL((k_eval_pf<NN, true>), pg, blk, 0, S(stream), c->d, x, smod, cnt, o, q);  \
-            else hipLaunchKernelGGL((k_eval_pf<NN, false>), pg, blk, 0, S(stream), c->d, x, smod, cnt, o, q);     \
-        } else if (mm) {                                                                                         \
+        if (mm) {                                                                                                \
             if (fast) hipLaunchKernelGGL((k_eval<NN, true, LD_MM>), grid, blk, 0, S(stream), c->d, x, smod, snum,  \
                                          cnt, o, q);                                                             \
             else hipLaunchKernelGGL((k_eval<NN, false, LD_MM>), grid, blk, 0, S(stream), c->d, x, smod, snum, cnt, \
@@ -1321,12 +1281,6 @@ grns_status grns_debug_rare_staging(grns_ctx *c, int on) {
 grns_status grns_debug_grid_cap(grns_ctx *c, int blocks_per_cu) {
     if (!c || blocks_per_cu < 0) return GRNS_ERR_ARG;
     c->grid_cap = blocks_per_cu;
-    return GRNS_OK;
-}
-
-grns_status grns_debug_stream_tiles(grns_ctx *c, int mode) {
-    if (!c || mode < 0 || mode > 2) return GRNS_ERR_ARG;
-    c->stream_mode = mode;
     return GRNS_OK;
 }
 

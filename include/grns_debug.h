@@ -37,12 +37,6 @@ grns_status grns_debug_stage_read(grns_ctx *ctx, int op, double *ms_total, long 
  * the reference's loops (src/rnsdiv.cuh, src/rnsscal.cuh) for every number instead of the word
  * forms, so that the two can be compared on the device. */
 grns_status grns_debug_reference_loops(grns_ctx *ctx, int on);
-/* Test hook: how the thread-per-number kernel of grns_eval gets its residues (modulus-major rows,
- * at most 2^29 numbers).  mode 0 (the default): straight into registers, one block per 256
- * numbers.  mode 1 (n = 16 and 32, rows aligned to 16 bytes): through double-buffered LDS tiles.
- * mode 2: a persistent grid whose lanes load their next number while evaluating the current one.
- * All modes give the same results; the hook lets them be compared and timed on the device. */
-grns_status grns_debug_stream_tiles(grns_ctx *ctx, int mode);
 /* Test hook: caps the grid of the per-number streaming kernels of grns_eval, grns_cmp and the mpint
  * operations (grid-stride loops) at blocks_per_cu blocks per compute unit; 0 restores one block per
  * 256 numbers.  Results do not depend on it; the hook lets grid shapes be timed on the device. */

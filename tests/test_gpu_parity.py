@@ -55,38 +55,6 @@ def test_eval_parity(ctxs, dev, n, fast):
     assert c.status() == 0
 
 
-@pytest.mark.parametrize("n", [16, 32])
-@pytest.mark.parametrize("fast", [False, True])
-@pytest.mark.parametrize("mode", [1, 2])
-def test_eval_streaming_forms(dev, n, fast, mode):
-    """The other streaming forms of eval (test hooks, off by default) equal the oracle: LDS-staged
-    tiles (mode 1, taken for 16-byte aligned rows whose stride is a multiple of 4) and the
-    persistent grid that prefetches each lane's next number (mode 2).  Whole blocks and a ragged
-    last one, with a packed row stride and with rows padded to a wider stride."""
-    import grns_amd as G
-    from oracle.oracle import Oracle
-    mod = G.largest_primes_below(n)
-    c, o = G.Context(mod, 0), Oracle(mod)
-    try:
-        c.stream_tiles(mode)
-        vals = eval_inputs(mod, 5 * 256 + 76, seed=40 + n)
-        x = to_res(vals, mod)
-        got = c.eval(_dev(x, dev), fast=fast)
-        assert got.cpu().numpy().tobytes() == o.eval_array(x, fast=fast).tobytes()
-        wide = np.zeros((n, x.shape[1] + 12), np.int32)
-        wide[:, :x.shape[1]] = x
-        ref = o.eval_array(x, fast=fast)
-        got = c.eval(_dev(wide, dev)[:, :x.shape[1]], fast=fast)
-        assert got.cpu().numpy().tobytes() == ref.tobytes()
-        # small inputs of a 16-moduli set can need a refinement factor past the reference's table:
-        # the status bit is raised exactly when the oracle met one (NaN bounds there)
-        past_table = np.isnan(ref["low"]["frac"]).any() or np.isnan(ref["upp"]["frac"]).any()
-        assert c.status() == (G.STATUS_REFINE_RANGE if past_table else 0)
-    finally:
-        c.close()
-        o.close()
-
-
 @pytest.mark.parametrize("n", [8, 32])
 def test_eval_number_major_layout(ctxs, dev, n):
     c, o, mod = ctxs[n]
