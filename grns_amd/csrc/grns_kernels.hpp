@@ -338,9 +338,11 @@ GRNS_D void eval_step(const DevCtx *__restrict__ c, const int *__restrict__ x, l
 
 // Plain per-call evaluation, streaming pass: one number per lane; numbers that need the
 // cooperative algorithm go to the rare-item queue with their residues.  Up to n = 32 the kernel
-// keeps 4 waves per SIMD (at most 128 VGPRs), as it has without the staged copy.
+// keeps 4 waves per SIMD (at most 128 VGPRs), as it has without the staged copy; at n = 64 it is
+// held to 2 (226 VGPRs, no spill; left to itself the compiler takes 256 and one wave per SIMD,
+// 0.78-0.86 ms for 2^23 numbers against 0.53 ms).
 template <int N, bool FAST, int MODE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(N <= 32 ? 4 : 1))) void k_eval(const DevCtx *__restrict__ c, const int *__restrict__ x, long long smod,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(N <= 32 ? 4 : 2))) void k_eval(const DevCtx *__restrict__ c, const int *__restrict__ x, long long smod,
                                                 long long snum, long long count, interval_t *__restrict__ out,
                                                 RareQ q) {
     for (long long base = (long long)blockIdx.x * BLOCK; base < count; base += (long long)gridDim.x * BLOCK) {

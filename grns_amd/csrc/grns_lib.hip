@@ -223,9 +223,7 @@ long long grid_elems(size_t count) {
     return b;
 }
 // The grid of a per-number streaming kernel (grid-stride loops): one block per BLOCK numbers, or at
-// most `cap` blocks per CU (the context's test cap when it sets one).  A kernel that fits one
-// block per CU (k_eval<64>: 256 VGPRs) runs 16% faster as a persistent grid of that many blocks
-// than as one block per 256 numbers (0.86 against 1.02 ms for 2^23 numbers).
+// most `cap` blocks per CU (the context's test cap when it sets one).
 long long grid_stream(const grns_ctx *c, size_t count, int cap = 0) {
     long long b = grid_elems(count);
     if (c->grid_cap > 0) cap = c->grid_cap;
@@ -519,7 +517,7 @@ grns_status grns_eval(grns_ctx *c, const int *x, long long smod, long long snum,
     if (count == 0) return GRNS_OK;
     DeviceGuard guard(c->device);
     interval_t *o = reinterpret_cast<interval_t *>(out);
-    dim3 grid((unsigned)grid_stream(c, count, c->n == 64 ? 1 : 0)), blk(BLOCK);
+    dim3 grid((unsigned)grid_stream(c, count)), blk(BLOCK);
     long long cnt = (long long)count;
     RareQ q;
     const bool mm = snum == 1 && count <= ((size_t)1 << 29);
