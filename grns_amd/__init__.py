@@ -83,7 +83,7 @@ DEBUG_EXPORTS = [
     "grns_debug_eval_full", "grns_debug_cmp_full",
     "grns_debug_mulmod_recip_d",
     "grns_debug_mulmod_recip", "grns_debug_addmod_recip", "grns_debug_bn_err", "grns_debug_fast_scalbn",
-    "grns_debug_div21",
+    "grns_debug_div32",
     "grns_debug_filter_tau", "grns_debug_filter_err", "grns_debug_stage_timing", "grns_debug_stage_read",
     "grns_debug_reference_loops", "grns_debug_grid_cap", "grns_debug_batch_per_call",
     "grns_debug_rare_staging", "grns_debug_rare_spread",
@@ -145,8 +145,8 @@ def lib():
     L.grns_debug_bn_err.argtypes = [D]
     L.grns_debug_fast_scalbn.restype = D
     L.grns_debug_fast_scalbn.argtypes = [D, I]
-    L.grns_debug_div21.restype = C.c_uint
-    L.grns_debug_div21.argtypes = [C.c_uint, C.c_uint, C.c_uint, C.POINTER(C.c_uint)]
+    L.grns_debug_div32.restype = C.c_uint
+    L.grns_debug_div32.argtypes = [C.c_uint] * 5 + [C.POINTER(C.c_uint)] * 3
     L.grns_debug_eval_full.restype = C.c_int
     L.grns_debug_eval_full.argtypes = [P, P, P, C.c_int, P]
     L.grns_debug_cmp_full.restype = C.c_int

@@ -217,26 +217,55 @@ GRNS_HD double fast_scalbn(double x, int n) {
     return bitsd(u & (1ull << 63));  // signed zero
 }
 
-// Division of a two-word number <u1, u0> = u1 2^32 + u0 by a normalised one-word d (top bit set,
-// u1 < d) with a precomputed reciprocal v = floor((2^64 - 1) / d) - 2^32: Moller and Granlund,
-// "Improved division by invariant integers", IEEE Trans. Computers 60(2), 2011, Algorithm 4.  One
-// product, a two-word addition and two conditional corrections, all integer: with wave-uniform
-// operands it runs on the scalar unit.  Returns the quotient (< 2^32); r gets the remainder.
+// The one-word reciprocal of a normalised d (top bit set): v = floor((2^64 - 1) / d) - 2^32, as in
+// Moller and Granlund, "Improved division by invariant integers", IEEE Trans. Computers 60(2), 2011.
 GRNS_HD unsigned recip_preinv(unsigned d) { return (unsigned)(0xFFFFFFFFFFFFFFFFull / d - (1ull << 32)); }
-GRNS_HD unsigned div21_preinv(unsigned u1, unsigned u0, unsigned d, unsigned v, unsigned &r) {
-    const unsigned long long p = (unsigned long long)v * u1 + (((unsigned long long)u1 << 32) | u0);
-    unsigned q1 = (unsigned)(p >> 32) + 1u;
+// Three words <u2, u1, u0> by a normalised two-word <d1, d0> (top bit of d1 set, <u2, u1> <
+// <d1, d0>) with v = floor((2^96 - 1) / <d1, d0>) - 2^32: Algorithm 5 of the same paper (one product
+// by v, two partial products and corrections, all integer: with wave-uniform operands it runs on
+// the scalar unit).  Returns floor(<u2, u1, u0> / <d1, d0>) (< 2^32); <r1, r0> gets the remainder.
+// recip32_preinv is the paper's Algorithm 6: v from the one-word reciprocal of d1, adjusted by d0.
+GRNS_HD unsigned recip32_preinv(unsigned d1, unsigned d0) {
+    unsigned v = recip_preinv(d1);
+    unsigned p = d1 * v + d0;  // mod 2^32
+    if (p < d0) {
+        v--;
+        if (p >= d1) {
+            v--;
+            p -= d1;
+        }
+        p -= d1;
+    }
+    const unsigned long long t = (unsigned long long)v * d0;
+    const unsigned t1 = (unsigned)(t >> 32), t0 = (unsigned)t;
+    p += t1;
+    if (p < t1) {
+        v--;
+        if (p > d1 || (p == d1 && t0 >= d0)) v--;
+    }
+    return v;
+}
+GRNS_HD unsigned div32_preinv(unsigned u2, unsigned u1, unsigned u0, unsigned d1, unsigned d0, unsigned v,
+                              unsigned &r1, unsigned &r0) {
+    const unsigned long long p = (unsigned long long)v * u2 + (((unsigned long long)u2 << 32) | u1);
+    unsigned q1 = (unsigned)(p >> 32);
     const unsigned q0 = (unsigned)p;
-    unsigned rr = u0 - q1 * d;  // mod 2^32
-    if (rr > q0) {
-        q1 -= 1u;
+    const unsigned long long d = ((unsigned long long)d1 << 32) | d0;
+    // <u1 - q1 d1, u0> - q1 d0 - d, mod 2^64
+    unsigned long long rr = (((unsigned long long)(u1 - q1 * d1) << 32) | u0) - (unsigned long long)d0 * q1 - d;
+    q1++;
+    if ((unsigned)(rr >> 32) >= q0) {
+        q1--;
         rr += d;
     }
-    if (rr >= d) {  // rare
-        q1 += 1u;
+    r1 = (unsigned)(rr >> 32);
+    r0 = (unsigned)rr;
+    if (r1 > d1 || (r1 == d1 && r0 >= d0)) {  // rare
+        q1++;
         rr -= d;
+        r1 = (unsigned)(rr >> 32);
+        r0 = (unsigned)rr;
     }
-    r = rr;
     return q1;
 }
 
@@ -728,19 +757,29 @@ GRNS_D unsigned long long gballot(const Group &g, bool p) {
 // gen and prop are disjoint.
 GRNS_D unsigned long long carry_in_bits(unsigned long long gen, unsigned long long prop) {
     const unsigned long long a = gen | prop;
-    return (a + gen) ^ a ^ gen;
+    return (a + gen) ^ prop;  // a ^ gen = prop
 }
 // The word of the next lower lane (0 into group lane 0), by a DPP wave shift.
+// A whole-wave group takes the shift's zero fill for lane 0 (bound_ctrl: no source lane reads 0).
 template <int GD>
 GRNS_D unsigned wlower(const Group &g, unsigned v) {
+    if constexpr (GD == 64) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, true);
     const unsigned r = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);  // wave_shr:1
     return g.gl == 0 ? 0u : r;
 }
 // The word of the next higher lane (0 into the top lane of the group).
 template <int GD>
 GRNS_D unsigned whigher(const Group &g, unsigned v) {
+    if constexpr (GD == 64) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, true);
     const unsigned r = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);  // wave_shl:1
     return g.gl == GD - 1 ? 0u : r;
+}
+// Bit gl of a group mask (group lane 0 at bit 0), on each lane: for a whole-wave group the mask
+// is read as a lane condition (one select), otherwise shifted out.
+template <int GD>
+GRNS_D unsigned lane_bit(const Group &g, unsigned long long mask) {
+    if constexpr (GD == 64) return (unsigned)__builtin_amdgcn_inverse_ballot_w64(mask);
+    else return (unsigned)((mask >> g.gl) & 1);
 }
 // Word idx (uniform in the group) of the group's value, on every lane of the group.
 template <int GD>
@@ -769,19 +808,28 @@ GRNS_D void wadd(const Group &g, unsigned &u, unsigned v) {
     const unsigned long long t = (unsigned long long)u + v;
     const unsigned w = (unsigned)t;
     const unsigned long long cin = carry_in_bits(gballot<GD>(g, (t >> 32) != 0), gballot<GD>(g, w == 0xffffffffu));
-    u = w + (unsigned)((cin >> g.gl) & 1);
+    u = w + lane_bit<GD>(g, cin);
 }
 // u -= q v (q < 2^32, the same on every lane of the group)
 template <int GD>
 GRNS_D void wmul_sub(const Group &g, unsigned &u, unsigned v, unsigned q) {
     const unsigned long long p = (unsigned long long)v * q;
     const unsigned hp = wlower<GD>(g, (unsigned)(p >> 32));
-    const long long t = (long long)u - (long long)(unsigned)p - (long long)hp;  // in (-2^33, 2^32)
-    const unsigned w = (unsigned)t;
-    const unsigned bp = wlower<GD>(g, (unsigned)(-(t >> 32)));  // borrow of 0, 1 or 2 from below
+    // u - low(p) - hp = w - (c0 + c1) 2^32
+    unsigned a, w;
+    const bool c0 = __builtin_add_overflow((unsigned)p, hp, &a);
+    const bool c1 = __builtin_sub_overflow(u, a, &w);
+    const unsigned bp = wlower<GD>(g, (unsigned)c0 + (unsigned)c1);  // borrow of 0, 1 or 2 from below
     const unsigned w2 = w - bp;
-    const unsigned long long bin = carry_in_bits(gballot<GD>(g, w < bp), gballot<GD>(g, w2 == 0));
-    u = w2 - (unsigned)((bin >> g.gl) & 1);
+    unsigned long long gen, prop;
+    if constexpr (GD == 64) {  // the compares straight into lane masks (ICMP_ULT, ICMP_EQ)
+        gen = __builtin_amdgcn_uicmp(w, bp, 36);
+        prop = __builtin_amdgcn_uicmp(w2, 0u, 32);
+    } else {
+        gen = gballot<GD>(g, w < bp);
+        prop = gballot<GD>(g, w2 == 0);
+    }
+    u = w2 - lane_bit<GD>(g, carry_in_bits(gen, prop));
 }
 
 // The value of canonical residues as words, by the Chinese remainder theorem:
@@ -803,7 +851,7 @@ GRNS_D unsigned crt_finish(const DevCtx *c, const Group &g, unsigned long long l
     const unsigned long long t = (unsigned long long)(unsigned)v + cp;
     const unsigned w = (unsigned)t;
     const unsigned long long cin = carry_in_bits(gballot<GD>(g, (t >> 32) != 0), gballot<GD>(g, w == 0xffffffffu));
-    unsigned X = w + (unsigned)((cin >> g.gl) & 1);
+    unsigned X = w + lane_bit<GD>(g, cin);
     const unsigned mw = wl ? c->crt_words[(size_t)g.L * WW + g.gl] : 0u;
     wmul_sub<GD>(g, X, mw, (unsigned)kf);
     if (!(fr > 0x1p-30 && fr < 1.0 - 0x1p-30)) {
@@ -818,32 +866,56 @@ GRNS_D unsigned crt_finish(const DevCtx *c, const Group &g, unsigned long long l
     return X;
 }
 
+// acc[k] += (word i0 + b of row s[k]) t[b] for the terms i0 + b < lim of one batch, with the wraps of
+// each 64-bit sum counted (LIM: the batch may run past lim; otherwise all of it is in range).
+template <int GD, int K, bool LIM>
+GRNS_D void word_terms(const Group &g, const int (&s)[K], const unsigned (&t)[WORDS_BATCH], int i0, int lim,
+                       unsigned long long (&acc)[K], unsigned (&wraps)[K]) {
+#pragma unroll
+    for (int b = 0; b < WORDS_BATCH; b++) {
+        if (LIM && i0 + b >= lim) break;  // the same on every lane
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const unsigned long long p = (unsigned long long)gword<GD>(g, (unsigned)s[k], i0 + b) * t[b];
+            const unsigned long long r = acc[k] + p;
+            wraps[k] += r < p;
+            acc[k] = r;
+        }
+    }
+}
+
 // The column sums of K numbers at once: s[k] on lane gl < L is the k-th number's s_gl; word gl of
 // M_i is loaded once for all K (8 rows of the table per batch of global loads, so that its latency
-// is paid once per 8 terms).
+// is paid once per 8 terms).  Each column is a 64-bit sum and a count of its wraps (at most L); lanes
+// at or above the word count read word 0 of the table and are cleared at the end.
 template <int GD, int K>
 GRNS_D void crt_columns(const DevCtx *c, const Group &g, const int (&s)[K], unsigned long long (&lo)[K],
                         unsigned long long (&hi)[K]) {
     const int WW = c->words;
     const bool wl = g.gl < WW;
     const global_uint *tw = (const global_uint *)(uintptr_t)c->crt_words + (wl ? g.gl : 0);
+    unsigned long long acc[K];
+    unsigned wraps[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) lo[k] = hi[k] = 0;
-    for (int i0 = 0; i0 < g.L; i0 += WORDS_BATCH) {
+    for (int k = 0; k < K; k++) acc[k] = wraps[k] = 0;
+    int i0 = 0;
+    for (; i0 + WORDS_BATCH <= g.L; i0 += WORDS_BATCH) {  // whole batches: no bounds on the terms
+        unsigned t[WORDS_BATCH];
+#pragma unroll
+        for (int b = 0; b < WORDS_BATCH; b++) t[b] = tw[(size_t)(i0 + b) * WW];
+        word_terms<GD, K, false>(g, s, t, i0, g.L, acc, wraps);
+    }
+    if (i0 < g.L) {
         unsigned t[WORDS_BATCH];
 #pragma unroll
         for (int b = 0; b < WORDS_BATCH; b++) t[b] = i0 + b < g.L ? tw[(size_t)(i0 + b) * WW] : 0u;
+        word_terms<GD, K, true>(g, s, t, i0, g.L, acc, wraps);
+    }
+    // sum = lo + hi 2^32
 #pragma unroll
-        for (int b = 0; b < WORDS_BATCH; b++) {
-            if (i0 + b >= g.L) break;  // the same on every lane
-            const unsigned tb = wl ? t[b] : 0u;
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const unsigned long long p = (unsigned long long)gword<GD>(g, (unsigned)s[k], i0 + b) * tb;
-                lo[k] += (unsigned)p;
-                hi[k] += p >> 32;
-            }
-        }
+    for (int k = 0; k < K; k++) {
+        lo[k] = wl ? (unsigned)acc[k] : 0u;
+        hi[k] = wl ? (acc[k] >> 32) + ((unsigned long long)wraps[k] << 32) : 0u;
     }
 }
 
@@ -888,22 +960,25 @@ GRNS_D int group_words_residue(const DevCtx *c, const Group &g, unsigned v, int 
     const int n = c->n;
     // 2^(32 k) mod m_gl for 8 words at a time (loads issued together, as in group_words)
     const global_int *p2 = (const global_int *)(uintptr_t)c->pow2 + (act ? g.gl : 0);
-    unsigned long long lo = 0, hi = 0;
+    unsigned long long acc[1] = {0};  // the sum is acc + wraps 2^64 (lanes gl >= n read modulus 0's row)
+    unsigned wraps[1] = {0};
+    const int vv[1] = {(int)v};
     if (nw > (c->log2M >> 5) + 1) nw = (c->log2M >> 5) + 1;  // a value below M has no more words
-    for (int k0 = 0; k0 < nw; k0 += WORDS_BATCH) {
-        int t[WORDS_BATCH];
+    int k0 = 0;
+    for (; k0 + WORDS_BATCH <= nw; k0 += WORDS_BATCH) {
+        unsigned t[WORDS_BATCH];
 #pragma unroll
-        for (int k = 0; k < WORDS_BATCH; k++) t[k] = k0 + k < nw ? p2[(size_t)(32 * (k0 + k)) * n] : 0;
+        for (int k = 0; k < WORDS_BATCH; k++) t[k] = (unsigned)p2[(size_t)(32 * (k0 + k)) * n];
+        word_terms<GD, 1, false>(g, vv, t, k0, nw, acc, wraps);
+    }
+    if (k0 < nw) {
+        unsigned t[WORDS_BATCH];
 #pragma unroll
-        for (int k = 0; k < WORDS_BATCH; k++) {
-            if (k0 + k >= nw) break;  // the same on every lane
-            const unsigned wk = gword<GD>(g, v, k0 + k);
-            const unsigned long long p = (unsigned long long)wk * (unsigned)(act ? t[k] : 0);
-            lo += (unsigned)p;
-            hi += p >> 32;
-        }
+        for (int k = 0; k < WORDS_BATCH; k++) t[k] = k0 + k < nw ? (unsigned)p2[(size_t)(32 * (k0 + k)) * n] : 0u;
+        word_terms<GD, 1, true>(g, vv, t, k0, nw, acc, wraps);
     }
     if (!act) return 0;
+    const unsigned long long lo = (unsigned)acc[0], hi = (acc[0] >> 32) + ((unsigned long long)wraps[0] << 32);
     const int m = c->m[g.gl];
     const double dinv = c->dinv[g.gl];
     const long long rl = rem_trunc((long long)lo, m, dinv), rh = rem_trunc((long long)hi, m, dinv);
@@ -928,37 +1003,30 @@ GRNS_D void group_div_words(const DevCtx *c, const Group &g, int xi, int di, int
         D = (D << sh) | (wlower<GD>(g, D) >> (32 - sh));
         X = (X << sh) | (wlower<GD>(g, X) >> (32 - sh));
     }
-    const unsigned long long v1 = gword<GD>(g, D, wd - 1);
-    const unsigned long long v2 = wd >= 2 ? gword<GD>(g, D, wd - 2) : 0u;
-    const unsigned vinv = recip_preinv((unsigned)v1);  // v1 is normalised: its top bit is set
+    const unsigned v1 = gword<GD>(g, D, wd - 1);
+    const unsigned v2 = wd >= 2 ? gword<GD>(g, D, wd - 2) : 0u;
+    const unsigned vinv = recip32_preinv(v1, v2);  // <v1, v2> is normalised: the top bit of v1 is set
+    const unsigned long long V12 = ((unsigned long long)v1 << 32) | v2;
     const int top = wx - wd;
     unsigned Ds = wshift_up<GD>(g, D, top);
     unsigned Q = 0;
     for (int j = top; j >= 0; j--) {
-        const unsigned long long u2 = gword<GD>(g, X, j + wd), u1 = gword<GD>(g, X, j + wd - 1);
-        const unsigned long long u0 = wd >= 2 ? gword<GD>(g, X, j + wd - 2) : 0u;
-        const unsigned long long num = (u2 << 32) | u1;
-        unsigned long long qh, rh;
-        if (u2 >= v1) {
-            qh = 0xffffffffull;
-            rh = num - qh * v1;
-        } else {  // the exact floor of <u2, u1> / v1 and its remainder
-            unsigned r32;
-            qh = div21_preinv((unsigned)u2, (unsigned)u1, (unsigned)v1, vinv, r32);
-            rh = r32;
+        const unsigned u2 = gword<GD>(g, X, j + wd), u1 = gword<GD>(g, X, j + wd - 1);
+        // with one divisor word (v2 = 0) u0 does not change the estimate: any lane will do
+        const unsigned u0 = gword<GD>(g, X, (j + wd - 2) & (GD - 1));
+        // qh = floor(<u2, u1, u0> / <v1, v2>): Knuth's estimate of step D3 with its test on v2 made
+        // exact, at most one above the quotient word.  It would reach 2^32 only for <u2, u1> =
+        // <v1, v2>, where the quotient word is 2^32 - 1 or 2^32 - 2 and 2^32 - 1 is taken.
+        unsigned r1, r0;
+        const unsigned qe = div32_preinv(u2, u1, u0, v1, v2, vinv, r1, r0);
+        const unsigned qh = (((unsigned long long)u2 << 32) | u1) < V12 ? qe : 0xffffffffu;
+        wmul_sub<GD>(g, X, Ds, qh);  // qh = 0 leaves X as it is
+        unsigned qd = qh;
+        if (wneg<GD>(g, X)) {
+            qd--;
+            wadd<GD>(g, X, Ds);
         }
-        while (rh < (1ull << 32) && qh * v2 > ((rh << 32) | u0)) {
-            qh--;
-            rh += v1;
-        }
-        if (qh) {
-            wmul_sub<GD>(g, X, Ds, (unsigned)qh);
-            if (wneg<GD>(g, X)) {
-                qh--;
-                wadd<GD>(g, X, Ds);
-            }
-        }
-        if (g.gl == j) Q = (unsigned)qh;
+        if (lane_bit<GD>(g, 1ull << j)) Q = qd;
         Ds = whigher<GD>(g, Ds);
     }
     const bool qnz = gballot<GD>(g, Q != 0) != 0;

@@ -1294,10 +1294,14 @@ int grns_debug_mulmod_recip(int x, int y, int m, double d) { return mulmod_recip
 int grns_debug_addmod_recip(int x, int y, int m, double d) { return addmod_recip(x, y, m, d); }
 double grns_debug_bn_err(double c) { return bn_err(c); }
 double grns_debug_fast_scalbn(double x, int n) { return fast_scalbn(x, n); }
-unsigned grns_debug_div21(unsigned u1, unsigned u0, unsigned d, unsigned *r) {
-    unsigned rr = 0;
-    const unsigned q = div21_preinv(u1, u0, d, recip_preinv(d), rr);
-    if (r) *r = rr;
+unsigned grns_debug_div32(unsigned u2, unsigned u1, unsigned u0, unsigned d1, unsigned d0, unsigned *r1,
+                          unsigned *r0, unsigned *v) {
+    const unsigned vv = recip32_preinv(d1, d0);
+    unsigned a = 0, b = 0;
+    const unsigned q = div32_preinv(u2, u1, u0, d1, d0, vv, a, b);
+    if (r1) *r1 = a;
+    if (r0) *r0 = b;
+    if (v) *v = vv;
     return q;
 }
 double grns_debug_filter_tau(const grns_ctx *c) { return c->h.tau; }

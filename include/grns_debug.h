@@ -22,9 +22,10 @@ int grns_debug_mulmod_recip(int x, int y, int m, double d);
 int grns_debug_addmod_recip(int x, int y, int m, double d);
 double grns_debug_bn_err(double c);
 double grns_debug_fast_scalbn(double x, int n);
-/* The word division's two-word by one-word step (d with its top bit set, u1 < d): quotient, and the
- * remainder in *r. */
-unsigned grns_debug_div21(unsigned u1, unsigned u0, unsigned d, unsigned *r);
+/* The word division's quotient estimate, a three-word by two-word step (d1 with its top bit set, <u2, u1> < <d1, d0>): quotient, the
+ * remainder in *r1, *r0 and the two-word reciprocal in *v. */
+unsigned grns_debug_div32(unsigned u2, unsigned u1, unsigned u0, unsigned d1, unsigned d0, unsigned *r1,
+                          unsigned *r0, unsigned *v);
 double grns_debug_filter_tau(const grns_ctx *ctx);
 double grns_debug_filter_err(const grns_ctx *ctx);
 /* Kernel timing: while enabled, grns_eval, grns_cmp and grns_max record HIP events on the caller's

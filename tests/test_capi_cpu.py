@@ -301,23 +301,29 @@ def test_rare_lane_routines_match_oracle(n):
     assert dc > 250
 
 
-def test_word_division_step_is_exact():
-    """The word division's 2/1 step (a reciprocal multiply and two corrections) returns
-    divmod(u1 2^32 + u0, d) for every normalised d and u1 < d: the extremes of d, u1 and u0 and
-    random values, checked against Python's integer division."""
+def test_word_division_estimate_is_exact():
+    """The quotient estimate of the word division (three words by a normalised two-word divisor,
+    with the two-word reciprocal floor((2^96 - 1) / d) - 2^32) returns divmod(u, d) for every
+    <u2, u1> < <d1, d0>: the extremes of d1, d0, the top of u and u0 and random values, checked with
+    Python's integers; the reciprocal is checked against its definition."""
     import ctypes as C
     L = G.lib()
-    rnd = random.Random(21)
+    rnd = random.Random(32)
     B = 1 << 32
-    ds = [1 << 31, (1 << 31) + 1, B - 1, B - 2, 3 << 30, 0xAAAAAAAB] + [rnd.randrange(1 << 31, B) for _ in range(300)]
-    r = C.c_uint()
+    d1s = [1 << 31, (1 << 31) + 1, B - 1, B - 2, 3 << 30] + [rnd.randrange(1 << 31, B) for _ in range(120)]
+    d0s = [0, 1, B - 1, B - 2, 1 << 31] + [rnd.randrange(B) for _ in range(3)]
+    r1, r0, v = C.c_uint(), C.c_uint(), C.c_uint()
     checked = 0
-    for d in ds:
-        u1s = [0, 1, d - 1, d - 2, d // 2] + [rnd.randrange(d) for _ in range(20)]
-        u0s = [0, 1, B - 1, B - 2, d - 1, d, (d + 1) % B] + [rnd.randrange(B) for _ in range(8)]
-        for u1 in u1s:
-            for u0 in u0s:
-                q = L.grns_debug_div21(u1, u0, d, C.byref(r))
-                assert (q, r.value) == divmod(u1 * B + u0, d), (u1, u0, d)
-                checked += 1
+    for d1 in d1s:
+        for d0 in d0s:
+            d = d1 * B + d0
+            tops = [0, 1, d - 1, d - 2, d // 2, d1 * B, (d1 - 1) * B + B - 1] + [rnd.randrange(d) for _ in range(12)]
+            tops = [t for t in tops if t < d]
+            u0s = [0, 1, B - 1, d0, (d0 + 1) % B] + [rnd.randrange(B) for _ in range(3)]
+            for top in tops:
+                for u0 in u0s:
+                    q = L.grns_debug_div32(top >> 32, top & (B - 1), u0, d1, d0, C.byref(r1), C.byref(r0), C.byref(v))
+                    assert (q, r1.value * B + r0.value) == divmod(top * B + u0, d), (top, u0, d1, d0)
+                    checked += 1
+            assert v.value == (B ** 3 - 1) // d - B, (d1, d0)
     assert checked > 100000
