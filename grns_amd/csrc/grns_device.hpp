@@ -866,6 +866,40 @@ GRNS_D unsigned crt_finish(const DevCtx *c, const Group &g, unsigned long long l
     return X;
 }
 
+// One term of a column sum on a whole-wave group: word `lane` of sv broadcast (v_readlane), its
+// product by t added into the 64-bit sum by the multiply-add itself (v_mad_u64_u32 with the sum as
+// addend) and that addition's carry-out counted (v_addc_co_u32): three vector instructions where
+// the compiler emits six (it does not use the multiply-add's carry-out).  A scalar register
+// written by a vector instruction is read by another after two wait states (s_nop 1).
+GRNS_D void term_wave(unsigned sv, int lane, unsigned t, unsigned long long &acc, unsigned &wraps) {
+    unsigned sw;
+    unsigned long long cy;
+    asm("v_readlane_b32 %[sw], %[sv], %[lane]\n\t"
+        "s_nop 1\n\t"
+        "v_mad_u64_u32 %[acc], %[cy], %[sw], %[t], %[acc]\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32 %[wr], %[cy], %[wr], 0, %[cy]"
+        : [acc] "+v"(acc), [wr] "+v"(wraps), [sw] "=&s"(sw), [cy] "=&s"(cy)
+        : [sv] "v"(sv), [lane] "s"(lane), [t] "v"(t));
+}
+// The same for two numbers sharing the table word t.
+GRNS_D void terms_wave2(unsigned sa, unsigned sb, int lane, unsigned t, unsigned long long &acca, unsigned &wa,
+                        unsigned long long &accb, unsigned &wb) {
+    unsigned xa, xb;
+    unsigned long long ca, cb;
+    asm("v_readlane_b32 %[xa], %[sa], %[lane]\n\t"
+        "v_readlane_b32 %[xb], %[sb], %[lane]\n\t"
+        "s_nop 1\n\t"
+        "v_mad_u64_u32 %[aa], %[ca], %[xa], %[t], %[aa]\n\t"
+        "v_mad_u64_u32 %[ab], %[cb], %[xb], %[t], %[ab]\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32 %[wa], %[ca], %[wa], 0, %[ca]\n\t"
+        "v_addc_co_u32 %[wb], %[cb], %[wb], 0, %[cb]"
+        : [aa] "+v"(acca), [wa] "+v"(wa), [ab] "+v"(accb), [wb] "+v"(wb), [xa] "=&s"(xa), [xb] "=&s"(xb),
+          [ca] "=&s"(ca), [cb] "=&s"(cb)
+        : [sa] "v"(sa), [sb] "v"(sb), [lane] "s"(lane), [t] "v"(t));
+}
+
 // acc[k] += (word i0 + b of row s[k]) t[b] for the terms i0 + b < lim of one batch, with the wraps of
 // each 64-bit sum counted (LIM: the batch may run past lim; otherwise all of it is in range).
 template <int GD, int K, bool LIM>
@@ -874,6 +908,14 @@ GRNS_D void word_terms(const Group &g, const int (&s)[K], const unsigned (&t)[WO
 #pragma unroll
     for (int b = 0; b < WORDS_BATCH; b++) {
         if (LIM && i0 + b >= lim) break;  // the same on every lane
+        if constexpr (GD == 64 && K == 2) {
+            terms_wave2((unsigned)s[0], (unsigned)s[1], i0 + b, t[b], acc[0], wraps[0], acc[1], wraps[1]);
+            continue;
+        }
+        if constexpr (GD == 64 && K == 1) {
+            term_wave((unsigned)s[0], i0 + b, t[b], acc[0], wraps[0]);
+            continue;
+        }
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const unsigned long long p = (unsigned long long)gword<GD>(g, (unsigned)s[k], i0 + b) * t[b];
