@@ -219,7 +219,19 @@ GRNS_HD double fast_scalbn(double x, int n) {
 
 // The one-word reciprocal of a normalised d (top bit set): v = floor((2^64 - 1) / d) - 2^32, as in
 // Moller and Granlund, "Improved division by invariant integers", IEEE Trans. Computers 60(2), 2011.
-GRNS_HD unsigned recip_preinv(unsigned d) { return (unsigned)(0xFFFFFFFFFFFFFFFFull / d - (1ull << 32)); }
+// The quotient (2^64 - 1) / d lies in (2^32, 2^33], and its correctly rounded double 2^64 / d is
+// within 2^-20 of it, so its floor is off by at most one, which one product decides: (2^32 + v) d
+// >= 2^64 exactly when v d >= (2^32 - d) 2^32.  (A 64-bit integer division here is a long
+// sequence of scalar steps.)  Checked against the definition for every d in [2^31, 2^32).
+GRNS_HD unsigned recip_preinv(unsigned d) {
+    const double qe = 18446744073709551616.0 / (double)d;
+    unsigned long long v = (unsigned long long)qe - (1ull << 32);  // <= 2^32
+    const unsigned long long lim = (unsigned long long)(0u - d) << 32;
+    const unsigned long long t = v * d;  // < 2^64
+    if (t >= lim) v--;                   // (2^32 + v) d > 2^64 - 1
+    else if (t + d < lim) v++;           // (2^32 + v + 1) d <= 2^64 - 1
+    return (unsigned)v;
+}
 // Three words <u2, u1, u0> by a normalised two-word <d1, d0> (top bit of d1 set, <u2, u1> <
 // <d1, d0>) with v = floor((2^96 - 1) / <d1, d0>) - 2^32: Algorithm 5 of the same paper (one product
 // by v, two partial products and corrections, all integer: with wave-uniform operands it runs on
