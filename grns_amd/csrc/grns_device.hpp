@@ -415,6 +415,20 @@ GRNS_HD long long rem_trunc(long long v, int m, double dinv) {
     }
     return r;
 }
+// rem_trunc for a v whose double dv the caller has formed exactly (|v| < 2^32: the quotient fits
+// 32 bits, so it converts as an int).  Same value as rem_trunc(v, m, dinv).
+GRNS_HD long long rem_trunc_d(long long v, double dv, int m, double dinv) {
+    const int q = (int)trunc(dv * dinv);
+    long long r = v - (long long)q * m;
+    if (v >= 0) {
+        if (r < 0) r += m;
+        else if (r >= m) r -= m;
+    } else {
+        if (r > 0) r -= m;
+        else if (r <= -m) r += m;
+    }
+    return r;
+}
 // 3-argument mod_mul, src/modular.cuh:62-66
 GRNS_HD int mulmod3(int x, int y, int m, double dinv) { return (int)rem_trunc((long long)x * (long long)y, m, dinv); }
 // mod_psub, src/modular.cuh:52-56
@@ -1118,6 +1132,9 @@ GRNS_D int group_scale2pow_words(const DevCtx *c, const Group &g, int xi, unsign
     const unsigned steps = t + (dr ? 1u : 0u);
     int y = xi;
     bool bad = false;
+    // (2^j)^-1 mod m for the full steps (j = THRESH) and the last one (j = dr), loaded once
+    const int inv_full = act && t ? c->pow2_inv[THRESH - 1][g.gl] : 0;
+    const int inv_last = act && dr ? c->pow2_inv[dr - 1][g.gl] : 0;
     for (unsigned st = 0; st < steps; st++) {
         const long long b = (long long)st * THRESH;
         if (b >= xbits && group_all(g, !act || y == 0)) break;  // every later step maps 0 to 0
@@ -1132,9 +1149,11 @@ GRNS_D int group_scale2pow_words(const DevCtx *c, const Group &g, int xi, unsign
         if (act) {
             const int cc = mulmod_recip(y, w, m, dinv);
             bad |= (unsigned)cc >= (unsigned)m;
-            const long long rm = rem_trunc((long long)r, m, dinv);
-            const int mult = (int)rem_trunc((long long)y - rm + m, m, dinv);
-            const int inv = c->pow2_inv[j - 1][g.gl];
+            // rm = r % m and mult = (y - rm + m) % m as rem_trunc forms them: r < 2^30 and
+            // |y - rm + m| < 2^32 have exact doubles, formed from 32-bit pieces
+            const int rm = (int)rem_trunc_d((long long)r, (double)r, m, dinv);
+            const int mult = (int)rem_trunc_d((long long)y + (m - rm), (double)y + (double)(m - rm), m, dinv);
+            const int inv = st < t ? inv_full : inv_last;
             const int qi = (int)(((double)mult * (double)inv) * dinv);
             const long long v = (long long)mult * inv - (long long)qi * m;
             bad |= v < -2147483648ll || v > 2147483647ll;
