@@ -1021,9 +1021,10 @@ GRNS_D void group_words_pair(const DevCtx *c, const Group &g, int xi, int di, un
     D = crt_finish<GD>(c, g, lo[1], hi[1], sd);
 }
 
-// Residue modulo m_gl (lanes gl < n) of the value held in the first nw words.
+// Residue modulo m_gl (lanes gl < n) of the value held in the first nw words, nw <= log2M / 32 + 1
+// (RNS_POW2 has the rows 2^0 .. 2^log2M).
 template <int GD>
-GRNS_D int group_words_residue(const DevCtx *c, const Group &g, unsigned v, int nw) {
+GRNS_D int words_residue(const DevCtx *c, const Group &g, unsigned v, int nw) {
     const bool act = g.gl < g.L;
     const int n = c->n;
     // 2^(32 k) mod m_gl for 8 words at a time (loads issued together, as in group_words)
@@ -1031,7 +1032,6 @@ GRNS_D int group_words_residue(const DevCtx *c, const Group &g, unsigned v, int 
     unsigned long long acc[1] = {0};  // the sum is acc + wraps 2^64 (lanes gl >= n read modulus 0's row)
     unsigned wraps[1] = {0};
     const int vv[1] = {(int)v};
-    if (nw > (c->log2M >> 5) + 1) nw = (c->log2M >> 5) + 1;  // a value below M has no more words
     int k0 = 0;
     for (; k0 + WORDS_BATCH <= nw; k0 += WORDS_BATCH) {
         unsigned t[WORDS_BATCH];
@@ -1051,6 +1051,12 @@ GRNS_D int group_words_residue(const DevCtx *c, const Group &g, unsigned v, int 
     const double dinv = c->dinv[g.gl];
     const long long rl = rem_trunc((long long)lo, m, dinv), rh = rem_trunc((long long)hi, m, dinv);
     return (int)rem_trunc(rh * (long long)c->c32[g.gl] + rl, m, dinv);
+}
+// The same for a value below M (held in up to GD words).
+template <int GD>
+GRNS_D int group_words_residue(const DevCtx *c, const Group &g, unsigned v, int nw) {
+    if (nw > (c->log2M >> 5) + 1) nw = (c->log2M >> 5) + 1;  // a value below M has no more words
+    return words_residue<GD>(c, g, v, nw);
 }
 
 // x = q d + r for canonical residues with d != 0: long division of the words (TAOCP vol. 2,

@@ -1676,8 +1676,11 @@ __global__ __launch_bounds__(BLOCK) void k_div(const DevCtx *__restrict__ c, int
 }
 
 // ---------------------------------------------------------------- binary <-> RNS conversion
-// rns_from_binary (src/rnsbase.cuh:253-261): lane i of a group forms |x| mod m_i by Horner's rule
-// over the words, from the most significant (word k read once per group, GD words per load).
+// rns_from_binary (src/rnsbase.cuh:253-261): lane i of a group forms |x| mod m_i.  The words are
+// taken in chunks of kc = min(GD, log2M / 32) (word k of a chunk in lane k): a chunk's residue is
+// sum_k word_k (2^(32 k) mod m_i) from RNS_POW2, and the chunks, most significant first, are
+// combined by Horner's rule with the factor 2^(32 kc) mod m_i.  Sets whose M has fewer than 64 bits
+// take Horner's rule word by word.  Either way the result is |x| mod m_i, which is unique.
 template <int GD>
 __global__ __launch_bounds__(BLOCK) void k_from_binary(const DevCtx *__restrict__ c, const unsigned *__restrict__ w,
                                                        long long wst, long long wsn, int nwords,
@@ -1691,18 +1694,31 @@ __global__ __launch_bounds__(BLOCK) void k_from_binary(const DevCtx *__restrict_
     const int m = act ? c->m[g.gl] : 1;
     const double dinv = act ? c->dinv[g.gl] : 1.0;
     const long long c32 = act ? c->c32[g.gl] : 0;
+    const int kc = (c->log2M >> 5) < GD ? (c->log2M >> 5) : GD;
+    const long long F = act && kc >= 2 ? c->pow2[(size_t)(32 * kc) * c->n + g.gl] : 0;  // 2^(32 kc) mod m
     for (long long j0 = wave * P; j0 < count; j0 += nwaves * P) {
         const long long j = j0 + (threadIdx.x & 63) / GD;
         if (j >= count) continue;
         long long acc = 0;
-        const int top = (nwords - 1) / GD;
-        for (int ch = top; ch >= 0; ch--) {  // chunks of GD words, most significant first
-            const int k0 = ch * GD;
-            const int kn = nwords - k0 < GD ? nwords - k0 : GD;
-            const unsigned wk = g.gl < kn ? w[j * wsn + (long long)(k0 + g.gl) * wst] : 0u;
-            for (int k = kn - 1; k >= 0; k--) {
-                const unsigned v = gword<GD>(g, wk, k);
-                acc = rem_trunc(acc * c32 + (long long)v, m, dinv);
+        if (kc >= 2) {
+            const int top = (nwords - 1) / kc;
+            for (int ch = top; ch >= 0; ch--) {  // chunks of kc words, most significant first
+                const int k0 = ch * kc;
+                const int kn = nwords - k0 < kc ? nwords - k0 : kc;
+                const unsigned wk = g.gl < kn ? w[j * wsn + (long long)(k0 + g.gl) * wst] : 0u;
+                const int res = words_residue<GD>(c, g, wk, kn);
+                if (act) acc = rem_trunc(acc * F + res, m, dinv);
+            }
+        } else {
+            const int top = (nwords - 1) / GD;
+            for (int ch = top; ch >= 0; ch--) {  // chunks of GD words, most significant first
+                const int k0 = ch * GD;
+                const int kn = nwords - k0 < GD ? nwords - k0 : GD;
+                const unsigned wk = g.gl < kn ? w[j * wsn + (long long)(k0 + g.gl) * wst] : 0u;
+                for (int k = kn - 1; k >= 0; k--) {
+                    const unsigned v = gword<GD>(g, wk, k);
+                    acc = rem_trunc(acc * c32 + (long long)v, m, dinv);
+                }
             }
         }
         if (act) {
