@@ -895,8 +895,10 @@ GRNS_D unsigned crt_finish(const DevCtx *c, const Group &g, unsigned long long l
 // One term of a column sum on a whole-wave group: word `lane` of sv broadcast (v_readlane), its
 // product by t added into the 64-bit sum by the multiply-add itself (v_mad_u64_u32 with the sum as
 // addend) and that addition's carry-out counted (v_addc_co_u32): three vector instructions where
-// the compiler emits six (it does not use the multiply-add's carry-out).  A scalar register
-// written by a vector instruction is read by another after two wait states (s_nop 1).
+// the same sum written in C compiles to five (a separate 64-bit add and compare: the compiler does
+// not use the multiply-add's carry-out).  A scalar register written by a vector instruction is
+// read by another vector instruction after two wait states (s_nop 1), as the compiler itself
+// schedules them on this target.
 GRNS_D void term_wave(unsigned sv, int lane, unsigned t, unsigned long long &acc, unsigned &wraps) {
     unsigned sw;
     unsigned long long cy;
