@@ -896,13 +896,16 @@ GRNS_D unsigned crt_finish(const DevCtx *c, const Group &g, unsigned long long l
 // product by t added into the 64-bit sum by the multiply-add itself (v_mad_u64_u32 with the sum as
 // addend) and that addition's carry-out counted (v_addc_co_u32): three vector instructions where
 // the same sum written in C compiles to five (a separate 64-bit add and compare: the compiler does
-// not use the multiply-add's carry-out).  A scalar register written by a vector instruction is
-// read by another vector instruction after two wait states (s_nop 1), as the compiler itself
-// schedules them on this target.
+// not use the multiply-add's carry-out).  The wait states are those the compiler itself keeps on
+// this target: one between a vector write of a VGPR and v_readlane of it (s_nop 0 first, since the
+// word may have been written just before), two between a vector write of a scalar register and a
+// vector read of it (s_nop 1).  The sums it writes are read only by ordinary vector arithmetic
+// (the next term, the final split), which needs no wait after a vector write.
 GRNS_D void term_wave(unsigned sv, int lane, unsigned t, unsigned long long &acc, unsigned &wraps) {
     unsigned sw;
     unsigned long long cy;
-    asm("v_readlane_b32 %[sw], %[sv], %[lane]\n\t"
+    asm("s_nop 0\n\t"
+        "v_readlane_b32 %[sw], %[sv], %[lane]\n\t"
         "s_nop 1\n\t"
         "v_mad_u64_u32 %[acc], %[cy], %[sw], %[t], %[acc]\n\t"
         "s_nop 1\n\t"
@@ -915,7 +918,8 @@ GRNS_D void terms_wave2(unsigned sa, unsigned sb, int lane, unsigned t, unsigned
                         unsigned long long &accb, unsigned &wb) {
     unsigned xa, xb;
     unsigned long long ca, cb;
-    asm("v_readlane_b32 %[xa], %[sa], %[lane]\n\t"
+    asm("s_nop 0\n\t"
+        "v_readlane_b32 %[xa], %[sa], %[lane]\n\t"
         "v_readlane_b32 %[xb], %[sb], %[lane]\n\t"
         "s_nop 1\n\t"
         "v_mad_u64_u32 %[aa], %[ca], %[xa], %[t], %[aa]\n\t"
