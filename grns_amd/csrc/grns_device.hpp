@@ -419,15 +419,10 @@ GRNS_HD long long rem_trunc(long long v, int m, double dinv) {
 // 32 bits, so it converts as an int).  Same value as rem_trunc(v, m, dinv).
 GRNS_HD long long rem_trunc_d(long long v, double dv, int m, double dinv) {
     const int q = (int)trunc(dv * dinv);
-    long long r = v - (long long)q * m;
-    if (v >= 0) {
-        if (r < 0) r += m;
-        else if (r >= m) r -= m;
-    } else {
-        if (r > 0) r -= m;
-        else if (r <= -m) r += m;
-    }
-    return r;
+    const long long r = v - (long long)q * m;
+    // the result lies in [lo, lo + m): [0, m) for v >= 0, (-m, 0] for v < 0; r is at most one m out
+    const long long lo = v >= 0 ? 0 : 1 - (long long)m;
+    return r < lo ? r + m : (r >= lo + m ? r - m : r);
 }
 // 3-argument mod_mul, src/modular.cuh:62-66
 GRNS_HD int mulmod3(int x, int y, int m, double dinv) { return (int)rem_trunc((long long)x * (long long)y, m, dinv); }
@@ -1142,7 +1137,7 @@ GRNS_D int group_scale2pow_words(const DevCtx *c, const Group &g, int xi, unsign
     const int xbits = 32 * wlen<GD>(g, X);
     const unsigned t = D / THRESH, dr = D % THRESH;
     const unsigned steps = t + (dr ? 1u : 0u);
-    int y = xi;
+    int y = act ? xi : 0;  // lanes past n keep 0 throughout (m = 1, w = 0, inverses 0)
     bool bad = false;
     // (2^j)^-1 mod m for the full steps (j = THRESH) and the last one (j = dr), loaded once
     const int inv_full = act && t ? c->pow2_inv[THRESH - 1][g.gl] : 0;
@@ -1158,19 +1153,17 @@ GRNS_D int group_scale2pow_words(const DevCtx *c, const Group &g, int xi, unsign
             const unsigned hi = (sh && w0 + 1 < GD) ? gword<GD>(g, X, w0 + 1) : 0u;
             r = (sh ? (lo >> sh) | (hi << (32 - sh)) : lo) & ((1u << j) - 1u);
         }
-        if (act) {
-            const int cc = mulmod_recip(y, w, m, dinv);
-            bad |= (unsigned)cc >= (unsigned)m;
-            // rm = r % m and mult = (y - rm + m) % m as rem_trunc forms them: r < 2^30 and
-            // |y - rm + m| < 2^32 have exact doubles, formed from 32-bit pieces
-            const int rm = (int)rem_trunc_d((long long)r, (double)r, m, dinv);
-            const int mult = (int)rem_trunc_d((long long)y + (m - rm), (double)y + (double)(m - rm), m, dinv);
-            const int inv = st < t ? inv_full : inv_last;
-            const int qi = (int)(((double)mult * (double)inv) * dinv);
-            const long long v = (long long)mult * inv - (long long)qi * m;
-            bad |= v < -2147483648ll || v > 2147483647ll;
-            y = (int)v;
-        }
+        const int cc = mulmod_recip(y, w, m, dinv);
+        bad |= (unsigned)cc >= (unsigned)m;
+        // rm = r % m and mult = (y - rm + m) % m as rem_trunc forms them: r < 2^30 and
+        // |y - rm + m| < 2^32 have exact doubles, formed from 32-bit pieces
+        const int rm = (int)rem_trunc_d((long long)r, (double)r, m, dinv);
+        const int mult = (int)rem_trunc_d((long long)y + (m - rm), (double)y + (double)(m - rm), m, dinv);
+        const int inv = st < t ? inv_full : inv_last;
+        const int qi = (int)(((double)mult * (double)inv) * dinv);
+        const long long v = (long long)mult * inv - (long long)qi * m;
+        bad |= v < -2147483648ll || v > 2147483647ll;
+        y = (int)v;
     }
     replay_ok = !group_any(g, bad);
     return y;
