@@ -94,7 +94,8 @@ def test_div_parity(ctxs, dev, n, fast):
 
 def _div_cases(mod, count, seed):
     """Dividends and divisors that exercise the word division: random sizes, divisors of one word,
-    all-ones and power-of-2^32 words, quotients near word boundaries, d > x, d == x, x == M - 1."""
+    all-ones and power-of-2^32 words, quotients near word boundaries, capped quotient estimates,
+    d > x, d == x, x == M - 1."""
     from grns_amd import product
     M = product(mod)
     L = M.bit_length()
@@ -115,6 +116,15 @@ def _div_cases(mod, count, seed):
                 ds += [1, (1 << 32) - 1, v, v, (1 << 31) + 1]
     xs += [0, 5, 7, 7, M - 1, M - 1, M - 1, M // 2]
     ds += [1, 7, 7, 1, 1, M - 1, 2, M // 2 + 1]
+    # x = d 2^(32k) - 1 (quotient words 2^32 - 1, remainder d - 1): with three or more divisor
+    # words the running remainder's top two words equal the divisor's, where the three-by-two
+    # quotient estimate would reach 2^32 and is capped
+    for _ in range(8):
+        wd = rnd.randrange(3, max(4, (L - 1) // 32))
+        d = rnd.getrandbits(32 * wd) | 1 | (1 << (32 * wd - 1 - rnd.randrange(31)))
+        for k in (1, 2, 3):
+            if d * (1 << (32 * k)) - 1 < M:
+                xs.append(d * (1 << (32 * k)) - 1); ds.append(d)
     # quotients divisible by a modulus (the CPU path's reciprocal addition can return m_i there)
     for m in mod:
         for k in (1, 2, 12345):
