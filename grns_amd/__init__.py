@@ -85,7 +85,7 @@ DEBUG_EXPORTS = [
     "grns_debug_mulmod_recip", "grns_debug_addmod_recip", "grns_debug_bn_err", "grns_debug_fast_scalbn",
     "grns_debug_div32",
     "grns_debug_filter_tau", "grns_debug_filter_err", "grns_debug_stage_timing", "grns_debug_stage_read",
-    "grns_debug_reference_loops", "grns_debug_grid_cap", "grns_debug_batch_per_call",
+    "grns_debug_reference_loops", "grns_debug_grid_cap", "grns_debug_max_blocks", "grns_debug_batch_per_call",
     "grns_debug_rare_staging", "grns_debug_rare_spread",
 ]
 
@@ -160,6 +160,7 @@ def lib():
     L.grns_debug_stage_timing.argtypes = [P, I]
     L.grns_debug_reference_loops.argtypes = [P, I]
     L.grns_debug_grid_cap.argtypes = [P, I]
+    L.grns_debug_max_blocks.argtypes = [P, I]
     L.grns_debug_rare_staging.argtypes = [P, I]
     L.grns_debug_rare_spread.argtypes = [P, I]
     L.grns_debug_batch_per_call.argtypes = [P, P]
@@ -381,6 +382,10 @@ class Context:
     def grid_cap(self, blocks_per_cu=0):
         """Cap the streaming kernels' grids at blocks_per_cu blocks per CU, 0 for none (test hook)."""
         _check(lib().grns_debug_grid_cap(self._h, int(blocks_per_cu)), "grns_debug_grid_cap")
+
+    def max_blocks(self, nblocks=0):
+        """Stage-1 grid of the arg-max: nblocks blocks, 0 for the default (test hook)."""
+        _check(lib().grns_debug_max_blocks(self._h, int(nblocks)), "grns_debug_max_blocks")
 
     def rare_staging(self, on=True):
         """Stage queued numbers' residues for the rare pass (on, the default) or not (test hook)."""

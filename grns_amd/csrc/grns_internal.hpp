@@ -18,7 +18,8 @@ struct grns_ctx {
     double *d_mrc_d = nullptr;
     int *d_status = nullptr;
     unsigned *d_crt = nullptr;      // DevCtx::crt_words
-    int max_blocks = 2048;         // arg-max stage-1 grid: one resident wave of blocks on the device
+    int max_blocks = 2048;         // arg-max stage-1 grid, from the occupancy query (max_grid_for)
+    int max_blocks_occ = 2048;     // the same as set at context creation (test hooks restore it)
     // rare-item queues of the per-call eval / cmp kernels, one per stream (made on first use)
     struct RareQueue {
         void *stream;

@@ -42,6 +42,10 @@ grns_status grns_debug_reference_loops(grns_ctx *ctx, int on);
  * operations (grid-stride loops) at blocks_per_cu blocks per compute unit; 0 restores one block per
  * 256 numbers.  Results do not depend on it; the hook lets grid shapes be timed on the device. */
 grns_status grns_debug_grid_cap(grns_ctx *ctx, int blocks_per_cu);
+/* Test hook: runs stage 1 of the arg-max (grns_max, grns_max_noalloc, grns_max_mrc) on nblocks blocks (at most
+ * 8192, stage 2's capacity); 0 restores the default grid (from the occupancy query).
+ * Results do not depend on it; the hook lets stage-1 grids be timed on the device. */
+grns_status grns_debug_max_blocks(grns_ctx *ctx, int nblocks);
 /* Test hook: on = 0 stops the streaming kernels of grns_eval and grns_cmp from copying the residues
  * of the numbers they queue for the rare pass (which then reads them from the inputs); 1, the
  * default, restores it.  Results do not depend on it. */

@@ -362,6 +362,45 @@ def test_max_parity(ctxs, dev, n, case, mrc):
     assert (res.cpu().numpy() == rres).all()
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [16, 32, 64])
+def test_max_any_stage1_grid(ctxs, dev, n):
+    """The arg-max of a planted workload (one maximum, 200 values one below it) is the same for
+    every stage-1 grid: the default, small and odd grids, and one block per 256 numbers."""
+    import grns_amd as G
+    from oracle.oracle import Oracle
+    own = n not in ctxs
+    if own:
+        mod = G.largest_primes_below(n)
+        c, o = G.Context(mod, 0), Oracle(mod)
+    else:
+        c, o, mod = ctxs[n]
+    M = G.product(mod)
+    rnd = random.Random(90 + n)
+    count = 1 << 18
+    vals = [rnd.randrange(M // 2) for _ in range(count)]
+    idx = rnd.sample(range(count), 201)
+    vals[idx[0]] = M // 2 - 1
+    for k in idx[1:]:
+        vals[k] = M // 2 - 2
+    x = to_res(vals, mod)
+    xd = _dev(x, dev)
+    ridx, rres = o.max_array(x)
+    assert ridx == idx[0]
+    try:
+        for nb in (0, 1, 7, 1000, 2979, 1024):
+            c.max_blocks(nb)
+            for mrc in (False, True):
+                res, got = c.max(xd, mrc=mrc)
+                assert int(got.item()) == ridx, (nb, mrc)
+                assert (res.cpu().numpy() == rres).all(), (nb, mrc)
+    finally:
+        c.max_blocks(0)
+        if own:
+            c.close()
+            o.close()
+
+
 @pytest.mark.parametrize("n", [8, 32, 64])
 def test_max_noncanonical_residues(ctxs, dev, n):
     """Residues stand for their value mod m_i (some raised by m_i, some made negative): both
